@@ -1,0 +1,47 @@
+// lup_dispatch.hpp -- launch tables filled by the per-shard instantiation
+// units (lup_inst_uniform.hip compiled once per kernel variant and shard).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lup_kernel.hpp"
+
+namespace lufqhe {
+
+// Kernel variants compiled into the library.
+//   0: CROUT_REF pivots, fused FP64 arithmetic (production default)
+//   1: ABS pivots, fused FP64 arithmetic
+//   2: CROUT_REF pivots, the reference's own formulas (bit-exact parity mode)
+constexpr int kNumVariants = 3;
+constexpr int variant_policy(int v) { return v == 1 ? POLICY_ABS : POLICY_CROUT_REF; }
+constexpr int variant_arith(int v) { return v == 2 ? ARITH_REF : ARITH_FUSED; }
+inline int variant_of(int policy, int arith) {
+  if (policy == POLICY_CROUT_REF && arith == ARITH_FUSED) return 0;
+  if (policy == POLICY_ABS && arith == ARITH_FUSED) return 1;
+  if (policy == POLICY_CROUT_REF && arith == ARITH_REF) return 2;
+  return -1;
+}
+
+constexpr int kMaxN = 64;
+
+// Launch shape of one (variant, n) kernel: L lanes per matrix, WPB waves per
+// workgroup.  The kernel is compiled for exactly one shape.
+struct KernelShape {
+  int lanes;
+  int waves_per_block;
+};
+
+// Launches the (variant, n) kernel on `blocks` workgroups.
+using BatchLaunch = hipError_t (*)(const BatchArgs&, int64_t blocks,
+                                   hipStream_t stream);
+
+struct KernelEntry {
+  BatchLaunch launch;
+  KernelShape shape;
+  const void* kernel;  // for occupancy queries
+};
+
+extern KernelEntry g_kernels[kNumVariants][kMaxN + 1];
+
+}  // namespace lufqhe

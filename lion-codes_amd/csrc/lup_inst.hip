@@ -1,0 +1,54 @@
+// lup_inst.hip -- explicit kernel instantiations for one variant and one
+// shard of the sizes n = 1..64 (n % LUFQHE_SHARDS == LUFQHE_SHARD).  The
+// build compiles this file once per (variant, shard) so the 192 fully
+// unrolled kernels compile in parallel; each object registers its kernels in
+// g_kernels at load time.
+#include <utility>
+
+#include "lup_dispatch.hpp"
+#include "lup_kernel.hpp"
+
+#if !defined(LUFQHE_VARIANT) || !defined(LUFQHE_SHARD) || !defined(LUFQHE_SHARDS)
+#error "compile with -DLUFQHE_VARIANT=v -DLUFQHE_SHARD=s -DLUFQHE_SHARDS=k"
+#endif
+
+namespace lufqhe {
+namespace {
+
+constexpr int kPolicy = variant_policy(LUFQHE_VARIANT);
+constexpr int kArith = variant_arith(LUFQHE_VARIANT);
+constexpr int kWavesPerBlock = 4;
+
+template <int N>
+hipError_t launch(const BatchArgs& args, int64_t blocks, hipStream_t stream) {
+  constexpr int L = default_group_lanes<N>();
+  hipLaunchKernelGGL((lup_det_kernel<N, L, kPolicy, kArith, kWavesPerBlock>),
+                     dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0,
+                     stream, args);
+  return hipGetLastError();
+}
+
+template <int N>
+void register_one() {
+  if constexpr (N % LUFQHE_SHARDS == LUFQHE_SHARD) {
+    constexpr int L = default_group_lanes<N>();
+    KernelEntry& e = g_kernels[LUFQHE_VARIANT][N];
+    e.launch = &launch<N>;
+    e.shape = KernelShape{L, kWavesPerBlock};
+    e.kernel = reinterpret_cast<const void*>(
+        &lup_det_kernel<N, L, kPolicy, kArith, kWavesPerBlock>);
+  }
+}
+
+template <int... I>
+void register_all(std::integer_sequence<int, I...>) {
+  (register_one<I + 1>(), ...);
+}
+
+struct Registrar {
+  Registrar() { register_all(std::make_integer_sequence<int, kMaxN>{}); }
+};
+Registrar registrar;
+
+}  // namespace
+}  // namespace lufqhe

@@ -1,0 +1,115 @@
+"""Seeded synthetic workloads of the five benchmark configurations.
+
+All generators are deterministic functions of their seed.  Gaussian entries
+use torch's counter-based (Philox) generator, so a batch generated on the GPU
+is reproducible; numpy variants exist for small CPU-side cases.
+
+  C1  4096 x n=16, re,im ~ N(0,1)
+  C2  65536 x n=32, re,im ~ N(0,1)
+  C3  32768 FQHE matrices, n ~ U{10..40}: z_i uniform in the disk of radius
+      sqrt(2n), A_ij = z_i^j exp(-|z_i|^2/4); log|det| is known analytically
+  C4  200000 x n ~ U{8..64}, N(0,1), 5% near-singular (one row replaced by
+      alpha*row_a + beta*row_b + 1e-8*N(0,1))
+  C5  walker configurations of n in {8,16,24,32,48,64} FQHE particles (the
+      serving workload lives in serve.py)
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+
+def gaussian(count: int, n: int, seed: int, device="cpu") -> torch.Tensor:
+    """complex128 [count, n, n], re and im i.i.d. N(0,1)."""
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    re = torch.randn(count, n, n, generator=g, dtype=torch.float64, device=device)
+    im = torch.randn(count, n, n, generator=g, dtype=torch.float64, device=device)
+    return torch.complex(re, im)
+
+
+def gaussian_np(count: int, n: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((count, n, n)) + 1j * rng.standard_normal((count, n, n)))
+
+
+# --------------------------------------------------------------------------- FQHE
+def fqhe_positions(n: int, rng: np.random.Generator) -> np.ndarray:
+    """n particle positions uniform in the disk of radius sqrt(2n)."""
+    R = math.sqrt(2.0 * n)
+    u = rng.random(n)
+    v = rng.random(n)
+    return R * np.sqrt(u) * np.exp(2j * np.pi * v)
+
+
+def fqhe_matrix(z: np.ndarray) -> np.ndarray:
+    """A_ij = z_i^j exp(-|z_i|^2/4), j = 0..n-1 (complex128)."""
+    n = len(z)
+    powers = z[:, None] ** np.arange(n)[None, :]
+    return powers * np.exp(-np.abs(z) ** 2 / 4.0)[:, None]
+
+
+def fqhe_analytic(z: np.ndarray) -> tuple[float, float]:
+    """Exact log|det| and arg det of fqhe_matrix(z) (Vandermonde times the
+    Gaussian factors): sum_{i<j} log|z_j - z_i| - sum |z_i|^2/4 and
+    sum_{i<j} arg(z_j - z_i) mod 2pi."""
+    n = len(z)
+    iu, ju = np.triu_indices(n, 1)
+    d = z[ju] - z[iu]
+    la = float(np.sum(np.log(np.abs(d))) - np.sum(np.abs(z) ** 2) / 4.0)
+    ph = float(np.angle(np.prod(d / np.abs(d))))
+    return la, ph
+
+
+def fqhe_batch(count: int, n_lo: int, n_hi: int, seed: int):
+    """FQHE batch with n ~ U{n_lo..n_hi}: (flat complex128, ns, offs, z list)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_lo, n_hi + 1, size=count).astype(np.int32)
+    offs = np.zeros(count, np.int64)
+    offs[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+    flat = np.empty(int(np.sum(ns.astype(np.int64) ** 2)), np.complex128)
+    zs = []
+    for m in range(count):
+        z = fqhe_positions(int(ns[m]), rng)
+        zs.append(z)
+        flat[offs[m]:offs[m] + ns[m] ** 2] = fqhe_matrix(z).ravel()
+    return flat, ns, offs, zs
+
+
+# --------------------------------------------------------------------------- mixed
+def mixed_sizes(count: int, n_lo: int, n_hi: int, seed: int):
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_lo, n_hi + 1, size=count).astype(np.int32)
+    offs = np.zeros(count, np.int64)
+    offs[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+    return ns, offs
+
+
+def mixed_batch(count: int, seed: int, n_lo: int = 8, n_hi: int = 64,
+                near_singular: float = 0.05, device="cpu"):
+    """Packed mixed-size batch: (flat complex128 tensor, ns, offs, singular mask).
+
+    Matrix m occupies flat[offs[m] : offs[m] + ns[m]**2] (row major, ld = n).
+    A fraction `near_singular` of the matrices get one row replaced by
+    alpha*row_a + beta*row_b + 1e-8*N(0,1) (complex alpha, beta ~ N(0,1))."""
+    ns, offs = mixed_sizes(count, n_lo, n_hi, seed)
+    total = int(offs[-1] + int(ns[-1]) ** 2)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 1)
+    flat = torch.complex(torch.randn(total, generator=g, dtype=torch.float64, device=device),
+                         torch.randn(total, generator=g, dtype=torch.float64, device=device))
+    rng = np.random.default_rng(seed + 2)
+    sing = rng.random(count) < near_singular
+    idx = np.nonzero(sing)[0]
+    for m in idx:
+        n = int(ns[m])
+        r, a, b = rng.choice(n, size=3, replace=False)
+        alpha = complex(rng.standard_normal(), rng.standard_normal())
+        beta = complex(rng.standard_normal(), rng.standard_normal())
+        base = int(offs[m])
+        M = flat[base:base + n * n].view(n, n)
+        noise = torch.complex(torch.tensor(rng.standard_normal(n)), torch.tensor(rng.standard_normal(n)))
+        M[r] = alpha * M[a] + beta * M[b] + 1e-8 * noise.to(device)
+    return flat, ns, offs, sing

@@ -1,0 +1,208 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each restatement is checked against values fixed independently of it:
+mpmath determinants of the committed fixtures (60 digits), closed forms,
+brute-force Leibniz expansion, LAPACK (numpy) and the analytic FQHE
+(Vandermonde) form.  A dropped term, a wrong sign, a transposed operand or a
+wrong pivot rule fails at least one of these.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits, golden
+
+
+def rel_err_log(la, ph, la_ref, ph_ref):
+    """|det/det_ref - 1| from log forms."""
+    d = (np.asarray(la) - la_ref) + 1j * np.angle(np.exp(1j * (np.asarray(ph) - ph_ref)))
+    return np.abs(np.expm1(d))
+
+
+# ---------------------------------------------------------------- mpmath goldens
+@pytest.mark.parametrize("fixture,tol", [("gauss", 1e-12), ("ref_fixture16", 1e-12), ("fqhe", 1e-9)])
+def test_truth_and_crout_vs_mpmath(orc, fixture, tol):
+    g = golden(fixture)
+    for key in [k[2:] for k in g if k.startswith("A_")]:
+        A = g[f"A_{key}"]
+        t = orc.truth(A)
+        e_truth = rel_err_log(t["logabs"], t["phase"], g[f"logabs_{key}"], g[f"phase_{key}"])
+        assert np.all(t["info"] == 0)
+        assert e_truth.max() < 1e-15 if fixture != "fqhe" else e_truth.max() < 1e-12, (key, e_truth)
+        c = orc.crout_ref(A)
+        la, ph = orc.log_form(c["mant"], c["exp"])
+        e = rel_err_log(la, ph, g[f"logabs_{key}"], g[f"phase_{key}"])
+        assert e.max() < tol, (fixture, key, e.max())
+
+
+def test_near_singular_vs_mpmath(orc):
+    g = golden("near_singular")
+    for key in ("8", "16", "32"):
+        t = orc.truth(g[f"A_{key}"])
+        e = rel_err_log(t["logabs"], t["phase"], g[f"logabs_{key}"], g[f"phase_{key}"])
+        # cond ~ 1e8: long double keeps ~1e-19 * cond
+        assert e.max() < 1e-9, (key, e)
+
+
+def test_fqhe_analytic_form_matches_mpmath(wl):
+    g = golden("fqhe")
+    for key in ("10", "16", "24", "32", "40"):
+        for z, la_ref, ph_ref in zip(g[f"z_{key}"], g[f"logabs_{key}"], g[f"phase_{key}"]):
+            la, ph = wl.fqhe_analytic(z)
+            assert abs(la - la_ref) < 1e-11 * max(1.0, abs(la_ref))
+            assert abs(np.angle(np.exp(1j * (ph - ph_ref)))) < 1e-9
+
+
+def test_reference_fixture_complex64(orc):
+    """The reference's own input file at the reference's own precision."""
+    g = golden("ref_fixture16")
+    c = orc.crout_ref_c64(g["A32_16"])
+    assert np.all(c["info"] == 0)
+    ref = np.exp(g["logabs_16"] + 1j * g["phase_16"])
+    err = np.abs(c["det"].astype(np.complex128) / ref - 1)
+    assert err.max() < 1e-3  # FP32 Crout on 16x16 inputs in (0,1)
+    # the FP64 restatement on the same (exactly widened) inputs
+    d = orc.crout_ref(g["A32_16"].astype(np.complex128))
+    assert np.all(np.abs(d["det"] / ref - 1) < 1e-12)
+
+
+# ---------------------------------------------------------------- closed forms
+def _leibniz(A):
+    n = A.shape[0]
+    tot = 0j
+    for p in itertools.permutations(range(n)):
+        sgn = 1
+        for i in range(n):
+            for j in range(i + 1, n):
+                if p[i] > p[j]:
+                    sgn = -sgn
+        prod = 1 + 0j
+        for i in range(n):
+            prod *= complex(A[i, p[i]])
+        tot += sgn * prod
+    return tot
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_leibniz_brute_force(orc, n):
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((8, n, n)) + 1j * rng.standard_normal((8, n, n))
+    ref = np.array([_leibniz(a) for a in A])
+    for out in (orc.crout_ref(A), orc.mirror(A, "crout_ref", "fused"), orc.mirror(A, "abs", "fused")):
+        d = out["mant"] * np.exp2(out["exp"].astype(float))
+        assert np.all(np.abs(d / ref - 1) < 1e-12), n
+
+
+def test_triangular_diagonal_and_permutation(orc):
+    rng = np.random.default_rng(7)
+    n = 12
+    U = np.triu(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    diag = np.prod(np.diag(U))
+    P = np.eye(n)[rng.permutation(n)]
+    sgnP = np.linalg.det(P).real
+    for M, want in ((U, diag), (U.T.copy(), diag), (P, sgnP), (P @ U, sgnP * diag)):
+        for out in (orc.crout_ref(M), orc.mirror(M, "abs", "fused"), orc.mirror(M, "crout_ref", "fused")):
+            d = out["mant"][0] * 2.0 ** out["exp"][0]
+            assert abs(d / want - 1) < 1e-13
+
+
+def test_row_permutation_sign_and_scaling(orc):
+    """det(PA) = sgn(P) det(A); det(alpha A) = alpha^n det(A)."""
+    rng = np.random.default_rng(8)
+    n = 20
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    base = orc.truth(A)
+    perm = rng.permutation(n)
+    sgn = np.linalg.det(np.eye(n)[perm]).real
+    alpha = 0.3 - 1.7j
+    for M, dla, dph in ((A[perm], 0.0, 0.0 if sgn > 0 else np.pi),
+                        (alpha * A, n * np.log(abs(alpha)), n * np.angle(alpha))):
+        for out in (orc.crout_ref(M), orc.mirror(M, "abs", "fused")):
+            la, ph = orc.log_form(out["mant"], out["exp"])
+            e = rel_err_log(la, ph, base["logabs"] + dla, base["phase"] + dph)
+            assert e.max() < 1e-12
+
+
+def test_block_diagonal_product(orc):
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))
+    B = rng.standard_normal((9, 9)) + 1j * rng.standard_normal((9, 9))
+    M = np.zeros((16, 16), complex)
+    M[:7, :7] = A
+    M[7:, 7:] = B
+    want = np.linalg.det(A) * np.linalg.det(B)
+    out = orc.crout_ref(M)
+    assert abs(out["mant"][0] * 2.0 ** out["exp"][0] / want - 1) < 1e-13
+
+
+def test_lapack_cross_check_and_mirror_identity(orc, wl):
+    for n in (1, 2, 3, 7, 16, 33, 64):
+        A = wl.gaussian_np(64, n, seed=n)
+        ref = np.linalg.det(A)
+        cr = orc.crout_ref(A, pivots=True)
+        assert np.all(cr["info"] == 0)
+        assert np.all(np.abs(cr["det"] / ref - 1) < 1e-9)
+        # the plain product equals mant * 2^exp bit for bit when it is normal
+        assert np.array_equal(bits(cr["det"]), bits(np.ldexp(cr["mant"].real, cr["exp"]) + 1j * np.ldexp(cr["mant"].imag, cr["exp"])))
+        # right-looking mirror in the reference's arithmetic == restated Crout
+        mr = orc.mirror(A, "crout_ref", "ref")
+        assert np.array_equal(bits(mr["mant"]), bits(cr["mant"]))
+        assert np.array_equal(mr["exp"], cr["exp"])
+        assert np.array_equal(mr["piv"], cr["piv"])
+
+
+def test_pivot_rule_is_scaled_real_part_with_last_tie(orc):
+    """Hand-checked pivots: |Re|/rowscale decides, exact ties go to the last row."""
+    # row scales: row0 max|Re| = 4, row1 = 1, row2 = 2
+    A = np.array([[2 + 0j, 4, 0], [1 + 9j, 0, 0], [1, 2, 0.5]], complex)
+    # column 0 criteria: 2/4 = 0.5, 1/1 = 1 (large imaginary part ignored), 1/2 = 0.5
+    out = orc.crout_ref(A, pivots=True)
+    assert out["piv"][0, 0] == 1
+    # exact tie: both criteria 1 -> the last row wins
+    B = np.array([[1 + 0j, 0.5], [2, 1]], complex)
+    out = orc.crout_ref(B, pivots=True)
+    assert out["piv"][0, 0] == 1
+    # ABS policy uses the modulus: row 1 (|1+9i|) wins column 0 of A
+    m = orc.mirror(A, "abs", "fused")
+    assert m["piv"][0, 0] == 1
+    m2 = orc.mirror(np.array([[3 + 0j, 1], [0 + 3j, 1]]), "abs", "fused")
+    assert m2["piv"][0, 0] == 1  # |3| == |3i|: tie to the last row
+
+
+def test_status_codes(orc):
+    A = np.eye(4, dtype=complex)
+    A[2] = 1j * np.ones(4)  # row 2 has no real part: the reference exit(0)s
+    out = orc.crout_ref(A)
+    assert out["info"][0] == -3 and np.isnan(out["mant"][0])
+    m = orc.mirror(A, "crout_ref", "fused")
+    assert m["info"][0] == -3
+    m = orc.mirror(A, "abs", "fused")
+    assert m["info"][0] == 0
+    Z = np.ones((5, 5), complex)
+    Z[:, 0] = 0  # zero first column: zero pivot at step 0
+    for out in (orc.crout_ref(Z + 0), orc.mirror(Z, "abs", "fused"), orc.mirror(Z, "crout_ref", "ref")):
+        assert out["info"][0] == 1 and out["mant"][0] == 0
+
+
+def test_fp64_overflow_carried_in_exponent(orc, wl):
+    """FQHE determinants pass the float64 range near n = 33; mant/exp keep them."""
+    rng = np.random.default_rng(5)
+    z = wl.fqhe_positions(64, rng)
+    A = wl.fqhe_matrix(z)
+    la, ph = wl.fqhe_analytic(z)
+    assert la > 710  # beyond float64
+    t = orc.truth(A)
+    assert abs(t["logabs"][0] - la) < 1e-9 * la
+    for pol in ("abs", "crout_ref"):
+        m = orc.mirror(A, pol, "fused")
+        l2, p2 = orc.log_form(m["mant"], m["exp"])
+        assert abs(l2[0] - la) < 1e-8 * la, pol
+
+
+def test_pairsum_matches_products(orc, wl):
+    A = wl.gaussian_np(10, 6, seed=3)
+    d = np.linalg.det(A)
+    want = np.sum(d[0::2] * d[1::2])
+    assert abs(orc.crout_ref_pairsum(A) / want - 1) < 1e-12
