@@ -38,6 +38,7 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-ffp-contract=off",  # every rounding step of the kernels is explicit
+    "-mllvm", "-pragma-unroll-threshold=1000000",  # fully unroll the step loop up to n = 64
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}",

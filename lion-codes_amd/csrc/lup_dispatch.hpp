@@ -25,10 +25,12 @@ inline int variant_of(int policy, int arith) {
 
 constexpr int kMaxN = 64;
 
-// Launch shape of one (variant, n) kernel: L lanes per matrix, WPB waves per
-// workgroup.  The kernel is compiled for exactly one shape.
+// Launch shape of one (variant, n) kernel: `lanes` lanes per matrix split into
+// `lane_cols` lane-columns, WPB waves per workgroup.  The kernel is compiled
+// for exactly one shape.
 struct KernelShape {
   int lanes;
+  int lane_cols;
   int waves_per_block;
 };
 

@@ -22,7 +22,8 @@ constexpr int kWavesPerBlock = 4;
 template <int N>
 hipError_t launch(const BatchArgs& args, int64_t blocks, hipStream_t stream) {
   constexpr int L = default_group_lanes<N>();
-  hipLaunchKernelGGL((lup_det_kernel<N, L, kPolicy, kArith, kWavesPerBlock>),
+  constexpr int CG = default_lane_cols<N>();
+  hipLaunchKernelGGL((lup_det_kernel<N, L, CG, kPolicy, kArith, kWavesPerBlock, 1>),
                      dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0,
                      stream, args);
   return hipGetLastError();
@@ -32,11 +33,12 @@ template <int N>
 void register_one() {
   if constexpr (N % LUFQHE_SHARDS == LUFQHE_SHARD) {
     constexpr int L = default_group_lanes<N>();
+    constexpr int CG = default_lane_cols<N>();
     KernelEntry& e = g_kernels[LUFQHE_VARIANT][N];
     e.launch = &launch<N>;
-    e.shape = KernelShape{L, kWavesPerBlock};
+    e.shape = KernelShape{L, CG, kWavesPerBlock};
     e.kernel = reinterpret_cast<const void*>(
-        &lup_det_kernel<N, L, kPolicy, kArith, kWavesPerBlock>);
+        &lup_det_kernel<N, L, CG, kPolicy, kArith, kWavesPerBlock, 1>);
   }
 }
 

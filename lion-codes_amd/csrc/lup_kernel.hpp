@@ -30,6 +30,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace lufqhe {
 
 enum Policy { POLICY_CROUT_REF = 0, POLICY_ABS = 1 };
@@ -52,9 +55,11 @@ struct DetOut {
 // ---------------------------------------------------------------------------
 // cross-lane helpers
 // ---------------------------------------------------------------------------
+// Every pattern used here reads a valid source lane for every lane, so the
+// no-old-value form applies (no register has to be cleared first).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
@@ -140,71 +145,183 @@ __device__ __forceinline__ void renorm(double& x, double& y, int& e) {
 }
 
 // ---------------------------------------------------------------------------
-// One group's factorisation of one matrix whose row r is in a[0..N-1].
-// Shared by the uniform kernel and the mixed-size kernels.
+// Tile geometry.  A group of L lanes factors one matrix.  The lanes form
+// RG = L/CG lane-rows by CG lane-columns (lane = rb*CG + cb, so a lane-row is
+// CG adjacent lanes).  Lane (rb, cb) holds the R row slots rb + t*RG and the C
+// column slots cb + s*CG, i.e. rows and columns are dealt cyclically, so the
+// remaining work stays spread over the lanes as the factorisation shrinks.
+// CG = 1 is one row per lane; CG > 1 spreads each row over CG lanes, which
+// lets CG lanes write the pivot row at once and each broadcast element feed
+// R rows.
 // ---------------------------------------------------------------------------
+template <int N, int L, int CG>
+struct Tile {
+  static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L must be a power of two");
+  static_assert(CG == 1 || CG == 2 || CG == 4, "CG must be 1, 2 or 4");
+  static constexpr int RG = L / CG;
+  static constexpr int R = (N + RG - 1) / RG;
+  static constexpr int C = (N + CG - 1) / CG;
+  static constexpr int NPAD = C * CG;
+  static constexpr int G = 64 / L;
+};
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so the
+// body sees its index as a constant (static register indices, DPP controls).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+constexpr int quad_perm(int a, int b, int c, int d) {
+  return a | (b << 2) | (c << 4) | (d << 6);
+}
+
+// Value held by lane `src` of this lane's lane-row (CG adjacent lanes).  The
+// callers pass a constant once the step loop is unrolled, so the branches
+// fold to one DPP move per dword.
+template <int CG>
+__device__ __forceinline__ double lanerow_bcast(double v, int src) {
+  if constexpr (CG == 1) {
+    return v;
+  } else if constexpr (CG == 2) {
+    return src == 0 ? dpp_d<quad_perm(0, 0, 2, 2)>(v) : dpp_d<quad_perm(1, 1, 3, 3)>(v);
+  } else {
+    switch (src) {
+      case 0: return dpp_d<quad_perm(0, 0, 0, 0)>(v);
+      case 1: return dpp_d<quad_perm(1, 1, 1, 1)>(v);
+      case 2: return dpp_d<quad_perm(2, 2, 2, 2)>(v);
+      default: return dpp_d<quad_perm(3, 3, 3, 3)>(v);
+    }
+  }
+}
+
+// Max over the CG lanes of a lane-row.
+template <int CG>
+__device__ __forceinline__ double lanerow_max(double v) {
+  if constexpr (CG > 1) v = vmax_f64(v, dpp_d<kQuadXor1>(v));
+  if constexpr (CG > 2) v = vmax_f64(v, dpp_d<kQuadXor2>(v));
+  return v;
+}
+
 struct GroupResult {
   double mx, my;
   int e;
   int info;
-  int my_piv;  // the pivot position recorded at step k == r
 };
 
-template <int N, int L, int POLICY, int ARITH>
-__device__ __forceinline__ GroupResult factor_rows(double2 (&a)[N], int r,
-                                                   bool row_live,
-                                                   double2* __restrict__ bc) {
-  // Row scale s_i = max_j |Re a_ij| (LU-fqhe/src/LUCoreCPU.cu:145-160).
-  double s = 1.0, rs = 1.0;
-  int rej = -1;
-  if constexpr (POLICY == POLICY_CROUT_REF) {
-    s = fabs(a[0].x);
+// Factorisation of one matrix held by a group as a[R][C] (see Tile).  Rows
+// never move; each row slot carries its reference position.  piv_rec[q]
+// receives the pivot position chosen at step gl + q*L.
+template <int N, int L, int CG, int POLICY, int ARITH, int R, int C, int PR>
+__device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
+                                                   double2* __restrict__ bc,
+                                                   int (&piv_rec)[PR]) {
+  using T = Tile<N, L, CG>;
+  static_assert(R == T::R && C == T::C && PR == (N + L - 1) / L, "tile shape");
+  constexpr int RG = T::RG;
+  const int rb = gl / CG, cb = gl % CG;
+
+  int pos[R];
+  bool act[R];
 #pragma unroll
-    for (int j = 1; j < N; ++j) s = fmax(s, fabs(a[j].x));
-    const bool bad = row_live && (s < kRowScaleMin);
-    const int first_bad = group_max_i<L>(bad ? (63 - r) : -1);
-    if (first_bad >= 0) rej = 63 - first_bad;
-    if (!row_live || bad) s = 1.0;
-    rs = 1.0 / s;
+  for (int t = 0; t < R; ++t) {
+    pos[t] = rb + t * RG;
+    act[t] = pos[t] < N;
   }
 
-  bool act = row_live;
-  int pos = r;
+  // Row scale s_i = max_j |Re a_ij| (LU-fqhe/src/LUCoreCPU.cu:145-160).
+  double sc[R], rs[R];
+  int rej = -1;
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    sc[t] = 1.0;
+    rs[t] = 1.0;
+  }
+  if constexpr (POLICY == POLICY_CROUT_REF) {
+    int worst = -1;  // 255 - (first rejected row) over this lane's slots
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      double m = fabs(a[t][0].x);
+#pragma unroll
+      for (int s = 1; s < C; ++s)
+        if (cb + s * CG < N) m = vmax_f64(m, fabs(a[t][s].x));
+      m = lanerow_max<CG>(m);
+      const bool bad = act[t] && m < kRowScaleMin;
+      if (bad) worst = max(worst, 255 - pos[t]);
+      sc[t] = (act[t] && !bad) ? m : 1.0;
+      rs[t] = 1.0 / sc[t];
+    }
+    const int first_bad = group_max_i<L>(worst);
+    if (first_bad >= 0) rej = 255 - first_bad;
+  }
+
   double dx = 1.0, dy = 0.0, sg = 1.0;
-  int de = 0, info = 0, my_piv = -1;
+  int de = 0, info = 0;
 
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    // 1. criterion
-    double c;
-    if constexpr (POLICY == POLICY_CROUT_REF) {
-      if constexpr (ARITH == ARITH_FUSED)
-        c = fabs(a[k].x) * rs;
-      else
-        c = fabs(a[k].x) / s;
-    } else {
-      if constexpr (ARITH == ARITH_FUSED)
-        c = fma(a[k].x, a[k].x, a[k].y * a[k].y);
-      else
-        c = a[k].x * a[k].x + a[k].y * a[k].y;
-    }
-    c = fmin(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
-    c = act ? c : -1.0;
+    const int ck = k % CG, sk = k / CG;
 
-    // 2. exact arg-max, ties to the larger reference position
-    const double cmax = group_max_d<L>(c);
-    const int ppos = group_max_i<L>(c == cmax ? pos : -1);
-    const bool is_piv = (pos == ppos);
-
-    // 3. LDS broadcast of the pivot row
-    if (is_piv) {
+    // 1. criterion of this lane's candidates, best first (ties: larger pos)
+    double cbest = -1.0;
+    int pbest = -1;
 #pragma unroll
-      for (int j = k; j < N; ++j) bc[j] = a[j];
+    for (int t = 0; t < R; ++t) {
+      const double2 v = a[t][sk];
+      double c;
+      if constexpr (POLICY == POLICY_CROUT_REF) {
+        if constexpr (ARITH == ARITH_FUSED)
+          c = fabs(v.x) * rs[t];
+        else
+          c = fabs(v.x) / sc[t];
+      } else {
+        if constexpr (ARITH == ARITH_FUSED)
+          c = fma(v.x, v.x, v.y * v.y);
+        else
+          c = v.x * v.x + v.y * v.y;
+      }
+      c = fmin(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
+      c = (act[t] && cb == ck) ? c : -1.0;
+      if (t == 0) {
+        cbest = c;
+        pbest = pos[0];
+      } else {
+        const bool better = (c > cbest) || (c == cbest && pos[t] > pbest);
+        cbest = better ? c : cbest;
+        pbest = better ? pos[t] : pbest;
+      }
+    }
+
+    // 2. exact arg-max over the group
+    const double cmax = group_max_d<L>(cbest);
+    const int ppos = group_max_i<L>(cbest == cmax ? pbest : -1);
+    bool is_piv[R];
+    bool any_piv = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      is_piv[t] = (pos[t] == ppos);
+      any_piv = any_piv || is_piv[t];
+    }
+
+    // 3. the pivot's lane-row writes the pivot row (columns >= k) to LDS
+    if (any_piv) {
+#pragma unroll
+      for (int s = sk; s < C; ++s) {
+        double2 v = a[0][s];
+#pragma unroll
+        for (int t = 1; t < R; ++t)
+          if (is_piv[t]) v = a[t][s];
+        bc[cb + s * CG] = v;
+      }
     }
     wave_lds_sync();
     const double2 p = bc[k];
 
-    if (r == k) my_piv = ppos;
+    if (gl == k % L) piv_rec[k / L] = ppos;
     if (ppos != k) sg = -sg;
 
     // 4. determinant product
@@ -222,7 +339,7 @@ __device__ __forceinline__ GroupResult factor_rows(double2 (&a)[N], int r,
       renorm(dx, dy, de);
     }
 
-    // 5. multipliers and rank-1 update
+    // 5. multipliers and rank-1 update of columns > k
     double den;
     if constexpr (ARITH == ARITH_FUSED)
       den = fma(p.x, p.x, p.y * p.y);
@@ -230,43 +347,95 @@ __device__ __forceinline__ GroupResult factor_rows(double2 (&a)[N], int r,
       den = p.x * p.x + p.y * p.y;
     const bool zero = (den == 0.0);
     if (zero && info == 0) info = k + 1;
-    if (k + 1 < N && act && !is_piv && !zero) {
-      double lx, ly;
+    if (k + 1 < N) {
+      double ir = 0.0, ii = 0.0;
       if constexpr (ARITH == ARITH_FUSED) {
-        const double rden = 1.0 / den;
-        const double ir = p.x * rden;
-        const double ii = -(p.y * rden);
-        lx = fma(a[k].x, ir, -(a[k].y * ii));
-        ly = fma(a[k].y, ir, a[k].x * ii);
-      } else {
-        lx = (a[k].x * p.x + a[k].y * p.y) / den;
-        ly = (a[k].y * p.x - a[k].x * p.y) / den;
+        const double rden = 1.0 / (zero ? 1.0 : den);
+        ir = p.x * rden;
+        ii = -(p.y * rden);
       }
+      const int s0 = (k + 1) / CG;  // first column slot with a column > k
+      // Every row slot is updated, with no divergent branch.  Finished rows
+      // and the pivot row are never read again after this step, so what the
+      // update leaves in them does not matter.  After a zero pivot the
+      // multipliers are exactly zero (ir = ii = 0 in fused arithmetic, an
+      // explicit zero in reference arithmetic), and fma(-0, u, e) == e for
+      // finite u up to the sign of a zero element, which no output sees.
+      double lx[R], ly[R];
 #pragma unroll
-      for (int j = k + 1; j < N; ++j) {
-        const double2 u = bc[j];
+      for (int t = 0; t < R; ++t) {
+        const double ax = lanerow_bcast<CG>(a[t][sk].x, ck);
+        const double ay = lanerow_bcast<CG>(a[t][sk].y, ck);
         if constexpr (ARITH == ARITH_FUSED) {
-          double x = fma(-lx, u.x, a[j].x);
-          x = fma(ly, u.y, x);
-          double y = fma(-lx, u.y, a[j].y);
-          y = fma(-ly, u.x, y);
-          a[j].x = x;
-          a[j].y = y;
+          lx[t] = fma(ax, ir, -(ay * ii));
+          ly[t] = fma(ay, ir, ax * ii);
         } else {
-          const double tx = lx * u.x - ly * u.y;
-          const double ty = ly * u.x + lx * u.y;
-          a[j].x = a[j].x - tx;
-          a[j].y = a[j].y - ty;
+          lx[t] = zero ? 0.0 : (ax * p.x + ay * p.y) / den;
+          ly[t] = zero ? 0.0 : (ay * p.x - ax * p.y) / den;
+        }
+      }
+      // Stream the pivot row through registers in chunks of CH columns,
+      // double buffered: chunk c+1 is read while chunk c is applied, and a
+      // scheduling barrier per chunk keeps the compiler from hoisting every
+      // read of the step (which would hold 4*(n-k) extra registers).
+      constexpr int CH = 4;
+      const int nch = (C - s0 + CH - 1) / CH;
+      double2 ub[2][CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        if (s0 + c < C) ub[0][c] = bc[cb + (s0 + c) * CG];
+#pragma unroll
+      for (int ci = 0; ci < nch; ++ci) {
+        const int c0 = s0 + ci * CH;
+        if (ci + 1 < nch) {
+#pragma unroll
+          for (int c = 0; c < CH; ++c)
+            if (c0 + CH + c < C) ub[(ci + 1) & 1][c] = bc[cb + (c0 + CH + c) * CG];
+        }
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            if (c0 + c >= C) continue;
+            const double2 u = ub[ci & 1][c];
+            double2& e = a[t][c0 + c];
+            if constexpr (ARITH == ARITH_FUSED) {
+              double x = fma(-lx[t], u.x, e.x);
+              x = fma(ly[t], u.y, x);
+              double y = fma(-lx[t], u.y, e.y);
+              y = fma(-ly[t], u.x, y);
+              e.x = x;
+              e.y = y;
+            } else {
+              const double tx = lx[t] * u.x - ly[t] * u.y;
+              const double ty = ly[t] * u.x + lx[t] * u.y;
+              e.x = e.x - tx;
+              e.y = e.y - ty;
+            }
+          }
+        }
+        // Pin this chunk's results before the next chunk's reads: the
+        // compiler may not sink these FMAs below, nor hoist later LDS reads
+        // above, this point.
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+#pragma unroll
+          for (int c = 0; c < CH; ++c)
+            if (c0 + c < C)
+              asm volatile("" : "+v"(a[t][c0 + c].x), "+v"(a[t][c0 + c].y)::"memory");
         }
       }
     }
     wave_lds_sync();
 
     // 6. the reference's swap, as position bookkeeping
-    if (pos == k) pos = ppos;
-    if (is_piv) {
-      pos = k;
-      act = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if (pos[t] == k) pos[t] = ppos;
+      if (is_piv[t]) {
+        pos[t] = k;
+        act[t] = false;
+      }
     }
   }
 
@@ -275,7 +444,6 @@ __device__ __forceinline__ GroupResult factor_rows(double2 (&a)[N], int r,
   res.my = sg * dy;
   res.e = de;
   res.info = (rej >= 0) ? -(rej + 1) : info;
-  res.my_piv = (rej >= 0) ? -1 : my_piv;
   return res;
 }
 
@@ -309,9 +477,9 @@ __device__ __forceinline__ void write_result(const DetOut& out, int64_t m,
 //   indexed addressing : A + offs[i] + r*N              (mixed-size buckets)
 // optionally with its rows gathered through row_perm (row r of matrix i is
 // source row row_perm[i*N + r]), and its results written to output slot
-// out_idx[i] (or i).  With next_task != nullptr the grid is persistent: after
-// its first task a wave takes task grid_tasks + atomicAdd(next_task, 1) until
-// the tasks run out, so every wave reaches the exit.
+// out_idx[i] (or i).  One task per wave: a loop over tasks would keep every
+// step's loop-invariant addresses live across iterations and double the
+// register footprint.
 // ---------------------------------------------------------------------------
 struct BatchArgs {
   const double2* A;
@@ -320,66 +488,85 @@ struct BatchArgs {
   const int64_t* offs;
   const int64_t* out_idx;
   const uint8_t* row_perm;
-  int32_t* next_task;
-  int64_t grid_tasks;
   int32_t piv_stride;
   DetOut out;
 };
 
-template <int N, int L, int POLICY, int ARITH, int WPB>
-__global__ void __launch_bounds__(64 * WPB) lup_det_kernel(BatchArgs args) {
-  static_assert(L >= N && L <= 64 && (L & (L - 1)) == 0, "bad group size");
-  constexpr int G = 64 / L;
-  constexpr int NP = N + 1;  // padded LDS slot: groups land on distinct banks
+template <int N, int L, int CG, int POLICY, int ARITH, int WPB, int OCC>
+__global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) {
+  using T = Tile<N, L, CG>;
+  constexpr int R = T::R, C = T::C, RG = T::RG, G = T::G;
+  constexpr int NP = T::NPAD + 1;  // padded LDS slot: groups on distinct banks
+  constexpr int PR = (N + L - 1) / L;
   __shared__ double2 bcast[WPB * G * NP];
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane / L;
-  const int r = lane % L;
+  const int gl = lane % L;
+  const int rb = gl / CG, cb = gl % CG;
   double2* bc = bcast + (wave * G + g) * NP;
-  const int64_t ntasks = (args.count + G - 1) / G;
-
-  int64_t task = (int64_t)blockIdx.x * WPB + wave;
-  while (task < ntasks) {
+  const int64_t task = (int64_t)blockIdx.x * WPB + wave;
+  if (task * G >= args.count) return;  // whole wave idle (uniform branch)
+  {
     const int64_t i = task * G + g;
     const bool mat_ok = i < args.count;
-    const bool row_live = mat_ok && r < N;
 
-    double2 a[N];
-    if (row_live) {
-      int src = r;
-      if (args.row_perm) src = min((int)args.row_perm[i * N + r], N - 1);
-      const double2* __restrict__ row =
-          args.offs ? args.A + args.offs[i] + (int64_t)src * N
-                    : args.A + i * args.stride + (int64_t)src * args.ld;
+    double2 a[R][C];
+    const double2* __restrict__ base = nullptr;
+    if (mat_ok) base = args.offs ? args.A + args.offs[i] : args.A + i * args.stride;
+    const int64_t ld = args.offs ? (int64_t)N : args.ld;
 #pragma unroll
-      for (int j = 0; j < N; ++j) a[j] = row[j];
-    } else {
+    for (int t = 0; t < R; ++t) {
+      const int row = rb + t * RG;
+      if (mat_ok && row < N) {
+        int src = row;
+        if (args.row_perm) src = min((int)args.row_perm[i * N + row], N - 1);
+        const double2* __restrict__ rp = base + (int64_t)src * ld;
 #pragma unroll
-      for (int j = 0; j < N; ++j) a[j] = make_double2(j == r ? 1.0 : 0.0, 0.0);
+        for (int s = 0; s < C; ++s) {
+          const int col = cb + s * CG;
+          a[t][s] = (col < N) ? rp[col] : make_double2(0.0, 0.0);
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < C; ++s)
+          a[t][s] = make_double2((cb + s * CG) == row ? 1.0 : 0.0, 0.0);
+      }
     }
 
+    int piv_rec[PR];
+#pragma unroll
+    for (int q = 0; q < PR; ++q) piv_rec[q] = -1;
     const GroupResult res =
-        factor_rows<N, L, POLICY, ARITH>(a, r, row_live, bc);
+        factor_tile<N, L, CG, POLICY, ARITH>(a, gl, bc, piv_rec);
 
     if (mat_ok) {
       const int64_t o = args.out_idx ? args.out_idx[i] : i;
-      if (args.out.piv && r < N)
-        args.out.piv[o * args.piv_stride + r] = (int8_t)res.my_piv;
-      if (r == 0) write_result(args.out, o, res);
+      if (args.out.piv) {
+#pragma unroll
+        for (int q = 0; q < PR; ++q) {
+          const int k = gl + q * L;
+          if (k < N)
+            args.out.piv[o * args.piv_stride + k] =
+                (int8_t)(res.info < 0 ? -1 : piv_rec[q]);
+        }
+      }
+      if (gl == 0) write_result(args.out, o, res);
     }
 
-    if (!args.next_task) break;
-    int32_t t = 0;
-    if (lane == 0) t = atomicAdd(args.next_task, 1);
-    task = args.grid_tasks + __builtin_amdgcn_readfirstlane(t);
   }
 }
 
+// Default launch shape per n: lanes per matrix and lane-columns.
 template <int N>
 constexpr int default_group_lanes() {
   return N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
 }
+template <int N>
+constexpr int default_lane_cols() {
+  return (N > 16 && N <= 40) ? 2 : 1;
+}
+
 
 }  // namespace lufqhe
