@@ -31,6 +31,10 @@ N, BATCH, SEED = 32, 65536, 20250101
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 OUT_BYTES = 16 + 4 + 8 + 8 + 4   # mant, exp, logabs, phase, info
+# HBM bytes one launch of the n=32 kernel moves, measured with rocprofv3
+# (FETCH_SIZE and WRITE_SIZE, one counter per run, each scaled by what the
+# same counter reads for a plain 1 GiB device copy: x2.0 and x1.0).
+MEASURED_TRAFFIC_N32 = 1.0772e9
 
 
 def det_bytes(n: int) -> int:
@@ -102,6 +106,9 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; a batch shards with no "
@@ -109,6 +116,14 @@ def main() -> None:
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 8, "launches": 3}))
+        return
     lc = importlib.import_module("lion-codes_amd")
     wl = importlib.import_module("lion-codes_amd.workloads")
     dev = torch.device("cuda:0")
@@ -157,7 +172,7 @@ def main() -> None:
                                f"one plain launch (config 2)", "batch": BATCH, "n": N,
                    "pivot_policy": "crout_ref", "arith": "fused", "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_N32,
                      "kernel_ms": kern_ms,
                      "fp64_tflops": flops_launch / (kern_ms * 1e-3) / 1e12,
                      "fp64_frac": flops_launch / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,

@@ -75,6 +75,43 @@ int32_t lufqhe_det_batched(const void *d_A, int64_t batch, int32_t n,
                            int32_t *d_info, int8_t *d_piv, int32_t policy,
                            int32_t arith, hipStream_t stream);
 
+/* Bytes of device workspace lufqhe_det_inplace needs for `count` matrices
+ * (one mantissa and one exponent per matrix). */
+int64_t lufqhe_det_inplace_workspace_bytes(int64_t count);
+
+/* The reference's in-place contract: LUCoreGPU::CALL built with WRITE_DET
+ * and PERFORM_PROD (LU-fqhe/Makefile:3, LU-fqhe/src/LUCoreGPU.cu:384-436):
+ * after the call A_m[0][0] holds det(M_m) as a plain complex128 (inf where
+ * it overflows, as the reference's product would) and, with pair_product,
+ * A_2i[0][0] holds det(M_2i) * det(M_2i+1).  Only the [0][0] entries are
+ * written (the reference also leaves its LU factors in A; nothing reads
+ * them).  d_work: lufqhe_det_inplace_workspace_bytes(count) bytes of device
+ * memory; if NULL the call allocates it stream-ordered (hipMallocAsync /
+ * hipFreeAsync on `stream`).  Arithmetic is FUSED. */
+int32_t lufqhe_det_inplace(void *d_A, int64_t count, int32_t n, int64_t ld,
+                           int64_t stride, int32_t pair_product,
+                           int32_t policy, void *d_work, hipStream_t stream);
+
+/* Bytes of device workspace the two pair sums need (fixed). */
+int64_t lufqhe_pair_sum_workspace_bytes(void);
+
+/* The reference's wavefunction sum, globalReduction
+ * (LU-fqhe/inc/LUCoreGPU.h:50, LU-fqhe/src/LUCoreGPU.cu:113-156,447-557):
+ * sum over i < count/2 of A_2i[0][0] (stride = matrix stride in complex
+ * elements), written to d_out[0] as complex128.  Deterministic (fixed
+ * summation order).  d_work: lufqhe_pair_sum_workspace_bytes() bytes, or
+ * NULL to allocate it stream-ordered. */
+int32_t lufqhe_global_reduction(const void *d_A, int64_t count, int64_t stride,
+                                void *d_work, void *d_out, hipStream_t stream);
+
+/* Overflow-free form of the same sum from (mant, exp) determinants:
+ * sum over i < count/2 of det_2i * det_2i+1 = out_mant * 2^out_exp
+ * (LU-fqhe/src/LUCoreCPU.cu:83-98 summed the pair products in float).
+ * Deterministic.  d_work as above. */
+int32_t lufqhe_pair_sum(const void *d_mant, const int32_t *d_exp,
+                        int64_t count, void *d_work, void *d_out_mant,
+                        int32_t *d_out_exp, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,5 +10,8 @@ from .lucore import (  # noqa: F401
     LufqheError,
     alloc_result,
     det_batched,
+    det_inplace,
+    global_reduction,
     load,
+    pair_sum,
 )

@@ -107,6 +107,19 @@ def build_library(force: bool = True, jobs: int | None = None, verbose: bool = F
     return LIB
 
 
+DRIVER_SRC = ROOT / "tests" / "cpp" / "lucore_gpu_driver.cpp"
+DRIVER = ROOT / "tests" / "cpp" / "lucore_gpu_driver"
+
+
+def build_driver(force: bool = True, verbose: bool = False) -> Path:
+    """C++ test driver of include/lucore_gpu.hpp, linked against the library."""
+    deps = [DRIVER_SRC, LIB, ROOT / "include" / "lucore_gpu.hpp", ROOT / "include" / "lufqhe.h"]
+    if force or _stale(DRIVER, deps):
+        _run([hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(DRIVER_SRC), "-o", str(DRIVER),
+              f"-L{PKG}", "-llufqhe", "-Wl,-rpath,$ORIGIN/../../lion-codes_amd"], verbose)
+    return DRIVER
+
+
 def build_oracle(force: bool = True, verbose: bool = False) -> Path:
     src = ORACLE_DIR / "lufqhe_oracle.c"
     deps = [src, ORACLE_DIR / "lufqhe_oracle.h"]
@@ -121,6 +134,7 @@ def build_oracle(force: bool = True, verbose: bool = False) -> Path:
 def build(force: bool = True, jobs: int | None = None, verbose: bool = False) -> None:
     build_oracle(force=force, verbose=verbose)
     build_library(force=force, jobs=jobs, verbose=verbose)
+    build_driver(force=force, verbose=verbose)
 
 
 if __name__ == "__main__":

@@ -18,6 +18,35 @@ int32_t hip_fail(hipError_t e) {
   return LUFQHE_ERROR_HIP;
 }
 
+hipError_t launch_inplace_write(double2* A, int64_t count, int64_t stride, const double2* mant,
+                                const int32_t* expo, int pair_product, hipStream_t stream);
+hipError_t launch_pair_sum_plain(const double2* A, int64_t pairs, int64_t stride,
+                                 double2* partial, double2* out, hipStream_t stream);
+hipError_t launch_pair_sum_scaled(const double2* mant, const int32_t* expo, int64_t pairs,
+                                  double2* part_m, int32_t* part_e, double2* out_m,
+                                  int32_t* out_e, hipStream_t stream);
+constexpr int64_t kPairSumParts = 1024;  // kSumMaxBlocks in lufqhe_epilogue.hip
+constexpr int64_t kPairSumWork = kPairSumParts * (16 + 4);
+
+// Device workspace given by the caller, or allocated stream-ordered.
+struct Workspace {
+  void* ptr = nullptr;
+  bool owned = false;
+  hipStream_t stream = nullptr;
+  hipError_t acquire(void* given, int64_t bytes, hipStream_t s) {
+    stream = s;
+    if (given) {
+      ptr = given;
+      return hipSuccess;
+    }
+    owned = true;
+    return hipMallocAsync(&ptr, (size_t)bytes, s);
+  }
+  ~Workspace() {
+    if (owned && ptr) (void)hipFreeAsync(ptr, stream);
+  }
+};
+
 // Workgroups for one plain (non-persistent) launch of `count` matrices.
 int64_t plain_blocks(const KernelEntry& e, int64_t count) {
   const int64_t per_wave = 64 / e.shape.lanes;
@@ -74,6 +103,58 @@ int32_t lufqhe_det_batched(const void* d_A, int64_t batch, int32_t n,
   args.out = DetOut{static_cast<double2*>(d_mant), d_exp, d_logabs, d_phase,
                     d_info, d_piv};
   const hipError_t err = e.launch(args, blocks, stream);
+  return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
+}
+
+int64_t lufqhe_det_inplace_workspace_bytes(int64_t count) {
+  return count <= 0 ? 0 : count * (16 + 4);
+}
+
+int32_t lufqhe_det_inplace(void* d_A, int64_t count, int32_t n, int64_t ld, int64_t stride,
+                           int32_t pair_product, int32_t policy, void* d_work,
+                           hipStream_t stream) {
+  if (count < 0 || n < 1 || n > kMaxN || ld < n || stride < 0) return LUFQHE_ERROR_INVALID_VALUE;
+  if (variant_of(policy, ARITH_FUSED) < 0) return LUFQHE_ERROR_UNSUPPORTED;
+  if (count == 0) return LUFQHE_SUCCESS;
+  if (!d_A) return LUFQHE_ERROR_INVALID_VALUE;
+  Workspace ws;
+  hipError_t err = ws.acquire(d_work, lufqhe_det_inplace_workspace_bytes(count), stream);
+  if (err != hipSuccess) return hip_fail(err);
+  double2* mant = static_cast<double2*>(ws.ptr);
+  int32_t* expo = reinterpret_cast<int32_t*>(mant + count);
+  const int32_t st = lufqhe_det_batched(d_A, count, n, ld, stride, mant, expo, nullptr, nullptr,
+                                        nullptr, nullptr, policy, ARITH_FUSED, stream);
+  if (st != LUFQHE_SUCCESS) return st;
+  err = launch_inplace_write(static_cast<double2*>(d_A), count, stride, mant, expo,
+                             pair_product, stream);
+  return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
+}
+
+int64_t lufqhe_pair_sum_workspace_bytes(void) { return kPairSumWork; }
+
+int32_t lufqhe_global_reduction(const void* d_A, int64_t count, int64_t stride, void* d_work,
+                                void* d_out, hipStream_t stream) {
+  if (count < 0 || stride < 0 || !d_out) return LUFQHE_ERROR_INVALID_VALUE;
+  if (count >= 2 && !d_A) return LUFQHE_ERROR_INVALID_VALUE;
+  Workspace ws;
+  hipError_t err = ws.acquire(d_work, kPairSumWork, stream);
+  if (err != hipSuccess) return hip_fail(err);
+  err = launch_pair_sum_plain(static_cast<const double2*>(d_A), count / 2, stride,
+                              static_cast<double2*>(ws.ptr), static_cast<double2*>(d_out), stream);
+  return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
+}
+
+int32_t lufqhe_pair_sum(const void* d_mant, const int32_t* d_exp, int64_t count, void* d_work,
+                        void* d_out_mant, int32_t* d_out_exp, hipStream_t stream) {
+  if (count < 0 || !d_out_mant || !d_out_exp) return LUFQHE_ERROR_INVALID_VALUE;
+  if (count >= 2 && (!d_mant || !d_exp)) return LUFQHE_ERROR_INVALID_VALUE;
+  Workspace ws;
+  hipError_t err = ws.acquire(d_work, kPairSumWork, stream);
+  if (err != hipSuccess) return hip_fail(err);
+  double2* part_m = static_cast<double2*>(ws.ptr);
+  int32_t* part_e = reinterpret_cast<int32_t*>(part_m + kPairSumParts);
+  err = launch_pair_sum_scaled(static_cast<const double2*>(d_mant), d_exp, count / 2, part_m,
+                               part_e, static_cast<double2*>(d_out_mant), d_out_exp, stream);
   return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
 }
 

@@ -300,22 +300,17 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
     const double cmax = group_max_d<L>(cbest);
     const int ppos = group_max_i<L>(cbest == cmax ? pbest : -1);
     bool is_piv[R];
-    bool any_piv = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) is_piv[t] = (pos[t] == ppos);
+
+    // 3. the pivot's lane-row writes the pivot row (columns >= k) to LDS:
+    // one exec-masked region per row slot, so no data selects are needed and
+    // a region whose mask is empty is branched over.
 #pragma unroll
     for (int t = 0; t < R; ++t) {
-      is_piv[t] = (pos[t] == ppos);
-      any_piv = any_piv || is_piv[t];
-    }
-
-    // 3. the pivot's lane-row writes the pivot row (columns >= k) to LDS
-    if (any_piv) {
+      if (is_piv[t]) {
 #pragma unroll
-      for (int s = sk; s < C; ++s) {
-        double2 v = a[0][s];
-#pragma unroll
-        for (int t = 1; t < R; ++t)
-          if (is_piv[t]) v = a[t][s];
-        bc[cb + s * CG] = v;
+        for (int s = sk; s < C; ++s) bc[cb + s * CG] = a[t][s];
       }
     }
     wave_lds_sync();

@@ -62,6 +62,16 @@ def load() -> ctypes.CDLL:
     lib.lufqhe_det_batched.restype = _i32
     lib.lufqhe_det_batched.argtypes = [_vp, _i64, _i32, _i64, _i64, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _i32, _i32, _vp]
+    lib.lufqhe_det_inplace_workspace_bytes.restype = _i64
+    lib.lufqhe_det_inplace_workspace_bytes.argtypes = [_i64]
+    lib.lufqhe_det_inplace.restype = _i32
+    lib.lufqhe_det_inplace.argtypes = [_vp, _i64, _i32, _i64, _i64, _i32, _i32, _vp, _vp]
+    lib.lufqhe_pair_sum_workspace_bytes.restype = _i64
+    lib.lufqhe_pair_sum_workspace_bytes.argtypes = []
+    lib.lufqhe_global_reduction.restype = _i32
+    lib.lufqhe_global_reduction.argtypes = [_vp, _i64, _i64, _vp, _vp, _vp]
+    lib.lufqhe_pair_sum.restype = _i32
+    lib.lufqhe_pair_sum.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp, _vp]
     _lib = lib
     return lib
 
@@ -121,6 +131,43 @@ def det_batched(A: torch.Tensor, policy: str = "crout_ref", arith: str = "fused"
     if st != SUCCESS:
         raise LufqheError("lufqhe_det_batched", st)
     return res
+
+
+def det_inplace(A: torch.Tensor, pair_product: bool = True, policy: str = "crout_ref",
+                workspace: torch.Tensor | None = None,
+                stream: torch.cuda.Stream | None = None) -> None:
+    """The reference's in-place contract (LUCoreGPU::CALL with WRITE_DET and
+    PERFORM_PROD): A[m,0,0] = det(A[m]), A[2i,0,0] = det(A[2i]) * det(A[2i+1])."""
+    if A.dtype != torch.complex128 or not A.is_cuda or A.dim() != 3 or A.stride(2) != 1:
+        raise ValueError("A must be a complex128 CUDA tensor [B, n, n] with unit column stride")
+    B, n, _ = A.shape
+    st = load().lufqhe_det_inplace(_ptr(A), B, n, A.stride(1), A.stride(0), int(pair_product),
+                                   POLICIES[policy], _ptr(workspace), _stream_handle(stream))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_det_inplace", st)
+
+
+def global_reduction(A: torch.Tensor, workspace: torch.Tensor | None = None,
+                     stream: torch.cuda.Stream | None = None) -> torch.Tensor:
+    """sum_i A[2i,0,0] as one complex128 on the device (globalReduction)."""
+    out = torch.empty(1, dtype=torch.complex128, device=A.device)
+    st = load().lufqhe_global_reduction(_ptr(A), A.shape[0], A.stride(0), _ptr(workspace),
+                                        _ptr(out), _stream_handle(stream))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_global_reduction", st)
+    return out
+
+
+def pair_sum(res: DetResult, workspace: torch.Tensor | None = None,
+             stream: torch.cuda.Stream | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+    """sum_i det_2i * det_2i+1 = mant * 2**exp, from a DetResult (overflow free)."""
+    mant = torch.empty(1, dtype=torch.complex128, device=res.mant.device)
+    exp = torch.empty(1, dtype=torch.int32, device=res.mant.device)
+    st = load().lufqhe_pair_sum(_ptr(res.mant), _ptr(res.exp), res.mant.shape[0],
+                                _ptr(workspace), _ptr(mant), _ptr(exp), _stream_handle(stream))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_pair_sum", st)
+    return mant, exp
 
 
 def logabs_phase_from(mant_re, mant_im, exp):

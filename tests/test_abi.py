@@ -56,3 +56,16 @@ def test_null_input_rejected(lc):
     lib = lc.load()
     assert lib.lufqhe_det_batched(None, 4, 8, 8, 64, None, None, None, None, None, None,
                                   0, 0, None) == 1
+
+
+def test_epilogue_argument_validation(lc):
+    lib = lc.load()
+    assert lib.lufqhe_det_inplace(None, 0, 8, 8, 64, 1, 0, None, None) == 0   # empty batch
+    assert lib.lufqhe_det_inplace(None, 4, 8, 8, 64, 1, 0, None, None) == 1   # null A
+    assert lib.lufqhe_det_inplace(None, 4, 0, 8, 64, 1, 0, None, None) == 1   # bad n
+    assert lib.lufqhe_det_inplace(None, 4, 8, 8, 64, 1, 9, None, None) == 2   # bad policy
+    assert lib.lufqhe_global_reduction(None, 4, 64, None, None, None) == 1     # null output
+    assert lib.lufqhe_global_reduction(None, 4, 64, None, 1, None) == 1        # null A, pairs > 0
+    assert lib.lufqhe_pair_sum(None, None, 4, None, None, None, None) == 1
+    assert lib.lufqhe_det_inplace_workspace_bytes(10) == 200
+    assert lib.lufqhe_pair_sum_workspace_bytes() > 0
