@@ -100,12 +100,158 @@ def accuracy(lc, A, res, count: int = 512) -> dict:
             "frac_le_1e-12": float(np.mean(e <= 1e-12))}
 
 
+def time_events(fn, stream, reps: int) -> float:
+    """Median milliseconds of fn() bracketed by HIP events on `stream`."""
+    import torch
+    ts = []
+    for _ in range(reps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        fn()
+        e.record(stream)
+        e.synchronize()
+        ts.append(s.elapsed_time(e))
+    return float(np.median(ts))
+
+
+def log_err_vs(la, ph, la_ref, ph_ref):
+    return np.abs(np.expm1((la - la_ref) + 1j * np.angle(np.exp(1j * (ph - ph_ref)))))
+
+
+def bench_mixed(lc, wl, dev, reps: int) -> dict:
+    """Config 4: 200000 matrices, n ~ U{8..64}, 5% near-singular; one
+    hipGraph replay of the bucketed plan vs per-size plain launches."""
+    import torch
+    from oracle import oracle
+    count, seed = 200000, 2024
+    flat, ns, offs, sing = wl.mixed_batch(count, seed, device=dev)
+    plan = lc.MixedPlan(ns, offs, nstreams=4)
+    res = lc.alloc_result(count, 64, dev)
+    side = torch.cuda.Stream()
+    out = {"workload": f"{count} complex128 matrices, n ~ U{{8..64}}, 5% near-singular "
+                       f"(one row = a*row_a + b*row_b + 1e-8 N(0,1)), packed, seed {seed}",
+           "buckets": len(plan.buckets())}
+    total_bytes = float(np.sum(16.0 * ns.astype(np.float64) ** 2 + OUT_BYTES))
+    total_flops = float(np.sum([det_flops(int(n)) for n in ns]))
+    with torch.cuda.stream(side):
+        for mode in ("plain", "streams", "graph"):
+            plan.execute(flat, res, mode=mode, stream=side)   # warm-up / capture
+            ms = time_events(lambda: plan.execute(flat, res, mode=mode, stream=side), side, reps)
+            out[mode] = {"ms": ms, "det_per_s": count / (ms * 1e-3),
+                         "hbm_frac": total_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                         "fp64_frac": total_flops / (ms * 1e-3) / (FP64_PEAK_TFLOPS * 1e12)}
+        # the form a caller of a one-size-per-launch API needs: gather each
+        # size into its own uniform batch, then one launch per size
+        groups = []
+        for n in np.unique(ns):
+            idx = np.nonzero(ns == n)[0]
+            el = (offs[idx][:, None] + np.arange(int(n) * int(n))[None, :]).ravel()
+            groups.append((int(n), torch.from_numpy(el).to(dev), len(idx)))
+        outs = {n: lc.alloc_result(c, n, dev) for n, _, c in groups}
+
+        def repacked():
+            for n, el, c in groups:
+                lc.det_batched(flat[el].view(c, n, n), out=outs[n], stream=side)
+        repacked()
+        ms = time_events(repacked, side, reps)
+        out["repacked_per_size"] = {"ms": ms, "det_per_s": count / (ms * 1e-3)}
+    side.synchronize()
+    out["speedup_graph_vs_plain"] = out["plain"]["ms"] / out["graph"]["ms"]
+    out["speedup_graph_vs_repacked"] = out["repacked_per_size"]["ms"] / out["graph"]["ms"]
+    assert int((res.info != 0).sum()) == 0
+    # accuracy on a sample against the long-double truth, by matrix class
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(count, 1000, replace=False))
+    sub_ns = ns[idx]
+    sub_offs = np.zeros(len(idx), np.int64)
+    sub_offs[1:] = np.cumsum(sub_ns.astype(np.int64) ** 2)[:-1]
+    sub = torch.cat([flat[int(offs[i]):int(offs[i]) + int(ns[i]) ** 2] for i in idx]).cpu().numpy()
+    t = oracle.truth_mixed(sub, sub_ns, sub_offs)
+    e = log_err_vs(res.logabs.cpu().numpy()[idx], res.phase.cpu().numpy()[idx],
+                   t["logabs"], t["phase"])
+    well = ~sing[idx]
+    out["accuracy"] = {"vs": "long-double LUP truth", "sample": len(idx),
+                       "median": float(np.median(e[well])), "p99": float(np.quantile(e[well], 0.99)),
+                       "frac_le_1e-12": float(np.mean(e[well] <= 1e-12)),
+                       "near_singular_median": float(np.median(e[~well])) if (~well).any() else None}
+    plan.close()
+    del flat
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_fqhe(lc, wl, dev, reps: int) -> dict:
+    """Config 3: 32768 FQHE matrices, n ~ U{10..40}, built on the device from
+    particle positions inside the determinant kernels."""
+    import torch
+    count, seed = 32768, 3
+    z, ns, offs = wl.fqhe_configs(count, 10, 40, seed=seed)
+    zd = torch.from_numpy(z).to(dev)
+    plan = lc.MixedPlan(ns, offs, nstreams=4)
+    res = lc.alloc_result(count, 64, dev)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        plan.execute(zd, res, mode="graph", stream=side, source="fqhe")
+        ms = time_events(lambda: plan.execute(zd, res, mode="graph", stream=side, source="fqhe"),
+                         side, reps)
+    side.synchronize()
+    la_ref = np.empty(count)
+    ph_ref = np.empty(count)
+    for m in range(count):
+        la_ref[m], ph_ref[m] = wl.fqhe_analytic(z[offs[m]:offs[m] + ns[m]])
+    la = res.logabs.cpu().numpy()
+    ph = res.phase.cpu().numpy()
+    dph = np.angle(np.exp(1j * (ph - ph_ref)))
+    rel = np.abs((la - la_ref) + 1j * dph) / np.abs(la_ref + 1j * ph_ref)
+    total_flops = float(np.sum([det_flops(int(n)) for n in ns]))
+    plan.close()
+    return {"workload": f"{count} FQHE matrices, n ~ U{{10..40}}, z uniform in the disk of "
+                        f"radius sqrt(2n), A_ij = z_i^j exp(-|z_i|^2/4) built in the kernel, "
+                        f"seed {seed}",
+            "ms": ms, "det_per_s": count / (ms * 1e-3),
+            "fp64_frac": total_flops / (ms * 1e-3) / (FP64_PEAK_TFLOPS * 1e12),
+            "analytic_log_form": {"median": float(np.median(rel)), "max": float(rel.max()),
+                                  "frac_le_1e-10": float(np.mean(rel <= 1e-10))}}
+
+
+def bench_serving(lc, dev) -> dict:
+    """Config 5: 512 walkers, n in {8,16,24,32,48,64}, k ~ U[1,2000] terms per
+    request, Poisson arrivals swept up to the measured capacity."""
+    sizes = [8, 16, 24, 32, 48, 64]
+    srv = lc.Server([sizes[w % 6] for w in range(512)], perms_per_walker=2000, seed=7)
+    common = dict(k_min=1, k_max=2000, max_batch_requests=128, max_inflight=4,
+                  hbm_budget=200 * 10 ** 9, records=False)
+    srv.run(requests=1024, rate=0.0, seed=1, **common)   # warm-up
+    cap, _ = srv.run(requests=8192, rate=0.0, seed=2, **common)
+    out = {"workload": "512 walkers (n = 8,16,24,32,48,64 in turn), FQHE entries, one particle "
+                       "moved per step (complex Gaussian, sigma 0.3), k ~ U[1,2000] "
+                       "row-permutation terms per request, 2000 permutations per walker, "
+                       "HBM budget 200 GB",
+           "capacity": {"det_per_s": cap["dets_per_s"], "requests_per_s": cap["requests_per_s"],
+                        "p50_ms": cap["p50_s"] * 1e3, "p99_ms": cap["p99_s"] * 1e3,
+                        "mean_batch_dets": cap["mean_batch_dets"]},
+           "sweep": []}
+    for frac in (0.25, 0.5, 0.8, 0.95):
+        rate = frac * cap["requests_per_s"]
+        nreq = int(max(512, min(60000, rate * 2.0)))
+        st, _ = srv.run(requests=nreq, rate=rate, seed=10 + int(frac * 100), **common)
+        out["sweep"].append({"offered_rate": rate, "load": frac, "requests": nreq,
+                             "det_per_s": st["dets_per_s"],
+                             "requests_per_s": st["requests_per_s"],
+                             "p50_ms": st["p50_s"] * 1e3, "p99_ms": st["p99_s"] * 1e3,
+                             "mean_batch_dets": st["mean_batch_dets"]})
+    srv.close()
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the mixed-size, FQHE and serving measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -182,6 +328,16 @@ def main() -> None:
     if not args.no_cpu:
         A_host = A[:16384].cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(A_host, host_cores())
+    if not args.no_extra:
+        del A
+        torch.cuda.empty_cache()
+        reps = max(3, min(args.steps, 10))
+        print("# mixed-size batch (config 4)", file=sys.stderr, flush=True)
+        out["mixed"] = bench_mixed(lc, wl, dev, reps)
+        print("# FQHE batch (config 3)", file=sys.stderr, flush=True)
+        out["fqhe"] = bench_fqhe(lc, wl, dev, reps)
+        print("# serving (config 5)", file=sys.stderr, flush=True)
+        out["serving"] = bench_serving(lc, dev)
     print(json.dumps(out), flush=True)
 
 

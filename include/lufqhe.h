@@ -75,6 +75,20 @@ int32_t lufqhe_det_batched(const void *d_A, int64_t batch, int32_t n,
                            int32_t *d_info, int8_t *d_piv, int32_t policy,
                            int32_t arith, hipStream_t stream);
 
+/* Launch shapes.  Every n has up to 3 compiled kernel shapes (lanes per
+ * matrix, lane-columns); the autotuner times them on the device and selects
+ * one per (policy, arithmetic, n).  Replaces the reference's fixed sizing,
+ * RunInfo::RunInfo (LU-fqhe/src/RunInfo.cpp:46-78) and the matrices-per-block
+ * command-line option (LU-fqhe/src/FQHE.cpp:214-265).  Selection is process
+ * wide; change it while no launch of that n is being issued. */
+int32_t lufqhe_shape_count(int32_t n);
+int32_t lufqhe_shape_info(int32_t n, int32_t shape, int32_t *lanes,
+                          int32_t *lane_cols);
+int32_t lufqhe_set_shape(int32_t policy, int32_t arith, int32_t n,
+                         int32_t shape);
+/* The selected shape, or -1 for invalid arguments. */
+int32_t lufqhe_get_shape(int32_t policy, int32_t arith, int32_t n);
+
 /* Bytes of device workspace lufqhe_det_inplace needs for `count` matrices
  * (one mantissa and one exponent per matrix). */
 int64_t lufqhe_det_inplace_workspace_bytes(int64_t count);
@@ -111,6 +125,152 @@ int32_t lufqhe_global_reduction(const void *d_A, int64_t count, int64_t stride,
 int32_t lufqhe_pair_sum(const void *d_mant, const int32_t *d_exp,
                         int64_t count, void *d_work, void *d_out_mant,
                         int32_t *d_out_exp, hipStream_t stream);
+
+/* ---- FQHE matrices built on the device ---------------------------------
+ * LU-fqhe's matrices are Laughlin-type wavefunction matrices
+ * A_ij = z_i^j exp(-|z_i|^2/4), j = 0..n-1, of particle positions z_i
+ * (complex128).  The reference builds them on the host and copies whole
+ * matrices to the device each Monte Carlo step (LU-fqhe/src/MonteCarloLoop.cu:
+ * 292-321; its on-device builder, LU-fqhe/src/mkMatrixElements.cu:169-358,
+ * is disabled).  Here the determinant kernels build each matrix from its
+ * positions in their load phase: 16n bytes read per matrix instead of 16n^2.
+ *
+ * Matrix i's positions start at d_z + i*z_stride (complex elements,
+ * z_stride >= n or 0 to share one configuration); with d_row_perm (n bytes
+ * per matrix, may be NULL) row r of matrix i is particle d_row_perm[i*n + r]
+ * (values above n-1 read as n-1).  Powers are sequential products
+ * z^j = z^(j-1) * z, so entries do not depend on the launch shape.
+ *
+ * lufqhe_fqhe_det: determinants as lufqhe_det_batched.
+ * lufqhe_fqhe_build: the same matrices written out (matrix i at
+ * d_A + i*n*n, ld = n), entry for entry what the fused load phase uses. */
+int32_t lufqhe_fqhe_det(const void *d_z, int64_t count, int32_t n,
+                        int64_t z_stride, const uint8_t *d_row_perm,
+                        void *d_mant, int32_t *d_exp, double *d_logabs,
+                        double *d_phase, int32_t *d_info, int8_t *d_piv,
+                        int32_t policy, int32_t arith, hipStream_t stream);
+int32_t lufqhe_fqhe_build(const void *d_z, int64_t count, int32_t n,
+                          int64_t z_stride, const uint8_t *d_row_perm,
+                          void *d_A, hipStream_t stream);
+
+/* ---- mixed-size batches ------------------------------------------------
+ * A batch whose matrix m has side n[m] (1..64) and starts at complex element
+ * offs[m] of one flat buffer (ld = n[m], row major).  The reference runs one
+ * size per RunInfo (LU-fqhe/src/RunInfo.cpp:46-78); a mixed batch there is
+ * one CALL per size on repacked uniform batches.
+ *
+ * lufqhe_mixed_plan_create buckets the batch by n on the host, uploads the
+ * per-bucket offset / output-slot tables (16 B per matrix, allocated here,
+ * freed by lufqhe_mixed_plan_destroy) and creates `nstreams` streams for the
+ * buckets, assigned largest-cost-first.  Fused arithmetic, the given pivot
+ * policy, the launch shape selected for each n.
+ *
+ * lufqhe_mixed_execute writes det outputs (as lufqhe_det_batched) at each
+ * matrix's own index m.  source LUFQHE_SOURCE_MATRIX reads matrix m at
+ * d_A + offs[m]; LUFQHE_SOURCE_FQHE reads n[m] particle positions there and
+ * builds the FQHE matrix on the device (as lufqhe_fqhe_det).  Modes:
+ *   LUFQHE_MIXED_GRAPH    the bucket launches forked over the plan streams,
+ *                         captured into one hipGraph on the first call for a
+ *                         given set of pointers (instantiation allocates) and
+ *                         replayed after; `stream` must not be the legacy
+ *                         default stream (capture is not allowed there);
+ *   LUFQHE_MIXED_STREAMS  the same forked launches, issued directly;
+ *   LUFQHE_MIXED_PLAIN    one launch per size, one after another on `stream`.
+ * A plan is not re-entrant: one execute at a time per plan. */
+#define LUFQHE_MIXED_GRAPH 0
+#define LUFQHE_MIXED_STREAMS 1
+#define LUFQHE_MIXED_PLAIN 2
+#define LUFQHE_MIXED_MAX_STREAMS 8
+#define LUFQHE_SOURCE_MATRIX 0
+#define LUFQHE_SOURCE_FQHE 1
+
+typedef struct lufqhe_mixed_plan lufqhe_mixed_plan;
+
+int32_t lufqhe_mixed_plan_create(const int32_t *h_n, const int64_t *h_offs,
+                                 int64_t count, int32_t policy,
+                                 int32_t nstreams, lufqhe_mixed_plan **plan);
+int32_t lufqhe_mixed_plan_destroy(lufqhe_mixed_plan *plan);
+/* Number of buckets; fills up to `capacity` of (n, count, stream) in launch
+ * order (NULL arrays are skipped).  -1 for a NULL plan. */
+int32_t lufqhe_mixed_plan_buckets(const lufqhe_mixed_plan *plan, int32_t *n,
+                                  int64_t *count, int32_t *stream,
+                                  int32_t capacity);
+int32_t lufqhe_mixed_execute(lufqhe_mixed_plan *plan, const void *d_A,
+                             void *d_mant, int32_t *d_exp, double *d_logabs,
+                             double *d_phase, int32_t *d_info, int32_t source,
+                             int32_t mode, hipStream_t stream);
+
+/* ---- serving: determinants for Monte Carlo walkers ----------------------
+ * The reference's Monte Carlo loop (LU-fqhe/src/MonteCarloLoop.cu:280-400)
+ * rebuilds all matrices on the host every iteration and runs one blocking
+ * CALL per GPU.  A server keeps `walkers` FQHE configurations (n[w]
+ * particles, uniform in the disk of radius sqrt(2n)) and, per walker, a table
+ * of `perms_per_walker` row permutations (its symmetrisation terms) on the
+ * device.  A request is one Monte Carlo step of one walker: move one particle
+ * by a complex Gaussian (sigma 0.3 per component), evaluate k permutation
+ * terms det(P_t A(z')) of the moved configuration and their sum psi', and
+ * apply the Metropolis rule |psi'|^2 >= u |psi|^2.
+ *
+ * lufqhe_server_run simulates `requests` Poisson arrivals at `rate` per
+ * second (rate <= 0: all at once); request r belongs to walker r mod walkers
+ * and k ~ U[k_min, k_max].  Requests are admitted while the HBM budget
+ * allows (resident state + batch buffers + 44 B per term in flight) and
+ * packed into batches of at most max_batch_requests / max_batch_dets;
+ * up to max_inflight batches run at once on their own streams, new ones
+ * launched as earlier ones are still running.  Random numbers depend only on
+ * (seed, request), so records are bit-identical for any rate or batching.
+ * Results per request go to `records` (may be NULL); the walkers keep their
+ * final configurations (lufqhe_server_reset restores the initial ones).
+ * The run allocates its batch buffers (hipMalloc, pinned host memory) and
+ * frees them before returning. */
+#define LUFQHE_SERVE_MAX_INFLIGHT 8
+
+typedef struct lufqhe_server lufqhe_server;
+
+typedef struct {
+  double rate;                /* requests per second; <= 0: all arrive at t = 0 */
+  int64_t requests;
+  int32_t k_min, k_max;       /* terms per request ~ U[k_min, k_max] */
+  uint64_t seed;
+  int32_t max_batch_requests; /* >= 1 */
+  int32_t max_inflight;       /* 1 .. LUFQHE_SERVE_MAX_INFLIGHT */
+  int64_t max_batch_dets;     /* >= k_max */
+  int64_t hbm_budget;         /* bytes */
+} lufqhe_serve_config;
+
+typedef struct {
+  int64_t requests, dets, batches;
+  double wall_s;              /* first arrival to last completion */
+  double dets_per_s, requests_per_s, offered_rate;
+  double p50_s, p99_s, max_s, mean_s; /* arrival to completion */
+  double mean_batch_dets;
+  int64_t budget_waits;       /* requests held back by the budget */
+  int64_t peak_inflight_bytes, resident_bytes;
+} lufqhe_serve_stats;
+
+typedef struct {
+  int32_t walker, particle, k, perm_start, accepted, sum_exp;
+  double delta[2];            /* the move */
+  double sum_mant[2];         /* psi' = sum_mant * 2^sum_exp */
+  double latency_s;
+} lufqhe_serve_record;
+
+int32_t lufqhe_server_create(int32_t walkers, const int32_t *h_walker_n,
+                             int32_t perms_per_walker, uint64_t seed,
+                             int32_t policy, lufqhe_server **server);
+int32_t lufqhe_server_destroy(lufqhe_server *server);
+int32_t lufqhe_server_reset(lufqhe_server *server);
+/* Walker w's n and (if h_z) its current configuration (n complex128). */
+int32_t lufqhe_server_walker(const lufqhe_server *server, int32_t walker,
+                             int32_t *n, void *h_z);
+/* Walker w's permutation table: perms_per_walker rows of 64 bytes (the
+ * first n used). */
+int32_t lufqhe_server_perms(const lufqhe_server *server, int32_t walker,
+                            uint8_t *h_perm);
+int32_t lufqhe_server_run(lufqhe_server *server,
+                          const lufqhe_serve_config *config,
+                          lufqhe_serve_stats *stats,
+                          lufqhe_serve_record *records);
 
 #ifdef __cplusplus
 }

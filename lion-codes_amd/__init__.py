@@ -8,10 +8,19 @@ from .lucore import (  # noqa: F401
     POLICIES,
     DetResult,
     LufqheError,
+    MixedPlan,
+    Server,
     alloc_result,
     det_batched,
+    apply_shape_table,
     det_inplace,
+    fqhe_build,
+    fqhe_det,
+    get_shape,
     global_reduction,
     load,
     pair_sum,
+    set_shape,
+    shape_count,
+    shape_info,
 )

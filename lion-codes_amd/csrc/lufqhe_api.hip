@@ -1,16 +1,33 @@
 // lufqhe_api.hip -- the C ABI (include/lufqhe.h) over the kernel tables.
 #include <limits.h>
 
+#include <atomic>
+
 #include "../../include/lufqhe.h"
 #include "lup_dispatch.hpp"
 
 namespace lufqhe {
 
 // Filled at load time by the lup_inst.hip objects.
-KernelEntry g_kernels[kNumVariants][kMaxN + 1];
+KernelEntry g_kernels[kNumVariants][kMaxN + 1][kMaxShapes];
 
 namespace {
 thread_local hipError_t t_last_hip = hipSuccess;
+std::atomic<int> g_selected[kNumVariants][kMaxN + 1];  // zero-initialised: shape 0
+}
+
+int selected_shape(int variant, int n) {
+  return g_selected[variant][n].load(std::memory_order_relaxed);
+}
+void set_selected_shape(int variant, int n, int shape) {
+  g_selected[variant][n].store(shape, std::memory_order_relaxed);
+}
+
+// The kernel a launch of (variant, n) uses: the selected shape if it is
+// built, else shape 0.
+const KernelEntry& entry_for(int variant, int n) {
+  const KernelEntry& e = g_kernels[variant][n][selected_shape(variant, n)];
+  return e.launch ? e : g_kernels[variant][n][0];
 }
 
 int32_t hip_fail(hipError_t e) {
@@ -25,6 +42,8 @@ hipError_t launch_pair_sum_plain(const double2* A, int64_t pairs, int64_t stride
 hipError_t launch_pair_sum_scaled(const double2* mant, const int32_t* expo, int64_t pairs,
                                   double2* part_m, int32_t* part_e, double2* out_m,
                                   int32_t* out_e, hipStream_t stream);
+hipError_t launch_fqhe_build(const double2* z, int64_t count, int n, int64_t z_stride,
+                             const uint8_t* row_perm, double2* out, hipStream_t stream);
 constexpr int64_t kPairSumParts = 1024;  // kSumMaxBlocks in lufqhe_epilogue.hip
 constexpr int64_t kPairSumWork = kPairSumParts * (16 + 4);
 
@@ -89,7 +108,7 @@ int32_t lufqhe_det_batched(const void* d_A, int64_t batch, int32_t n,
   if (v < 0) return LUFQHE_ERROR_UNSUPPORTED;
   if (batch == 0) return LUFQHE_SUCCESS;
   if (!d_A || !d_mant) return LUFQHE_ERROR_INVALID_VALUE;
-  const KernelEntry& e = g_kernels[v][n];
+  const KernelEntry& e = entry_for(v, n);
   if (!e.launch) return LUFQHE_ERROR_NOT_BUILT;
   const int64_t blocks = plain_blocks(e, batch);
   if (blocks > INT_MAX) return LUFQHE_ERROR_INVALID_VALUE;
@@ -104,6 +123,79 @@ int32_t lufqhe_det_batched(const void* d_A, int64_t batch, int32_t n,
                     d_info, d_piv};
   const hipError_t err = e.launch(args, blocks, stream);
   return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
+}
+
+int32_t lufqhe_fqhe_det(const void* d_z, int64_t count, int32_t n, int64_t z_stride,
+                        const uint8_t* d_row_perm, void* d_mant, int32_t* d_exp,
+                        double* d_logabs, double* d_phase, int32_t* d_info, int8_t* d_piv,
+                        int32_t policy, int32_t arith, hipStream_t stream) {
+  if (count < 0 || n < 1 || n > kMaxN || z_stride < 0 || (z_stride > 0 && z_stride < n))
+    return LUFQHE_ERROR_INVALID_VALUE;
+  const int v = variant_of(policy, arith);
+  if (v < 0) return LUFQHE_ERROR_UNSUPPORTED;
+  if (count == 0) return LUFQHE_SUCCESS;
+  if (!d_z || !d_mant) return LUFQHE_ERROR_INVALID_VALUE;
+  const KernelEntry& e = entry_for(v, n);
+  if (!e.launch) return LUFQHE_ERROR_NOT_BUILT;
+  const int64_t blocks = plain_blocks(e, count);
+  if (blocks > INT_MAX) return LUFQHE_ERROR_INVALID_VALUE;
+
+  BatchArgs args{};
+  args.A = static_cast<const double2*>(d_z);
+  args.count = count;
+  args.ld = n;
+  args.stride = z_stride;
+  args.row_perm = d_row_perm;
+  args.piv_stride = n;
+  args.src = SRC_FQHE;
+  args.out = DetOut{static_cast<double2*>(d_mant), d_exp, d_logabs, d_phase, d_info, d_piv};
+  const hipError_t err = e.launch(args, blocks, stream);
+  return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
+}
+
+int32_t lufqhe_fqhe_build(const void* d_z, int64_t count, int32_t n, int64_t z_stride,
+                          const uint8_t* d_row_perm, void* d_A, hipStream_t stream) {
+  if (count < 0 || n < 1 || n > kMaxN || z_stride < 0 || (z_stride > 0 && z_stride < n))
+    return LUFQHE_ERROR_INVALID_VALUE;
+  if (count == 0) return LUFQHE_SUCCESS;
+  if (!d_z || !d_A) return LUFQHE_ERROR_INVALID_VALUE;
+  if ((count * n + 255) / 256 > INT_MAX) return LUFQHE_ERROR_INVALID_VALUE;
+  const hipError_t err = launch_fqhe_build(static_cast<const double2*>(d_z), count, n, z_stride,
+                                           d_row_perm, static_cast<double2*>(d_A), stream);
+  return err == hipSuccess ? LUFQHE_SUCCESS : hip_fail(err);
+}
+
+int32_t lufqhe_shape_count(int32_t n) {
+  if (n < 1 || n > kMaxN) return 0;
+  int c = 0;
+  while (c < kMaxShapes && g_kernels[0][n][c].launch) ++c;
+  return c;
+}
+
+int32_t lufqhe_shape_info(int32_t n, int32_t shape, int32_t* lanes, int32_t* lane_cols) {
+  if (n < 1 || n > kMaxN || shape < 0 || shape >= kMaxShapes || !lanes || !lane_cols)
+    return LUFQHE_ERROR_INVALID_VALUE;
+  const KernelEntry& e = g_kernels[0][n][shape];
+  if (!e.launch) return LUFQHE_ERROR_NOT_BUILT;
+  *lanes = e.shape.lanes;
+  *lane_cols = e.shape.lane_cols;
+  return LUFQHE_SUCCESS;
+}
+
+int32_t lufqhe_set_shape(int32_t policy, int32_t arith, int32_t n, int32_t shape) {
+  const int v = variant_of(policy, arith);
+  if (v < 0) return LUFQHE_ERROR_UNSUPPORTED;
+  if (n < 1 || n > kMaxN || shape < 0 || shape >= kMaxShapes) return LUFQHE_ERROR_INVALID_VALUE;
+  if (!g_kernels[v][n][shape].launch) return LUFQHE_ERROR_NOT_BUILT;
+  set_selected_shape(v, n, shape);
+  return LUFQHE_SUCCESS;
+}
+
+int32_t lufqhe_get_shape(int32_t policy, int32_t arith, int32_t n) {
+  const int v = variant_of(policy, arith);
+  if (v < 0 || n < 1 || n > kMaxN) return -1;
+  const int sh = selected_shape(v, n);
+  return g_kernels[v][n][sh].launch ? sh : 0;
 }
 
 int64_t lufqhe_det_inplace_workspace_bytes(int64_t count) {

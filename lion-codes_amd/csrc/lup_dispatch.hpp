@@ -44,6 +44,12 @@ struct KernelEntry {
   const void* kernel;  // for occupancy queries
 };
 
-extern KernelEntry g_kernels[kNumVariants][kMaxN + 1];
+// g_kernels[variant][n][shape]: the fused variants carry every candidate
+// shape of candidate_shape(); the reference-arithmetic variant only shape 0.
+extern KernelEntry g_kernels[kNumVariants][kMaxN + 1][kMaxShapes];
+
+// Shape chosen per (variant, n); 0 until the autotuner's table is applied.
+int selected_shape(int variant, int n);
+void set_selected_shape(int variant, int n, int shape);
 
 }  // namespace lufqhe

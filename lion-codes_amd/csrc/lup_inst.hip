@@ -19,26 +19,33 @@ constexpr int kPolicy = variant_policy(LUFQHE_VARIANT);
 constexpr int kArith = variant_arith(LUFQHE_VARIANT);
 constexpr int kWavesPerBlock = 4;
 
-template <int N>
+template <int N, int SH>
 hipError_t launch(const BatchArgs& args, int64_t blocks, hipStream_t stream) {
-  constexpr int L = default_group_lanes<N>();
-  constexpr int CG = default_lane_cols<N>();
-  hipLaunchKernelGGL((lup_det_kernel<N, L, CG, kPolicy, kArith, kWavesPerBlock, 1>),
-                     dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0,
-                     stream, args);
+  constexpr ShapeLC sh = candidate_shape(N, SH);
+  hipLaunchKernelGGL((lup_det_kernel<N, sh.lanes, sh.lane_cols, kPolicy, kArith, kWavesPerBlock, 1>),
+                     dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0, stream, args);
   return hipGetLastError();
+}
+
+template <int N, int SH>
+void register_shape() {
+  constexpr ShapeLC sh = candidate_shape(N, SH);
+  // reference arithmetic is a parity mode: only its default shape is built
+  if constexpr (sh.lanes > 0 && (kArith == ARITH_FUSED || SH == 0)) {
+    KernelEntry& e = g_kernels[LUFQHE_VARIANT][N][SH];
+    e.launch = &launch<N, SH>;
+    e.shape = KernelShape{sh.lanes, sh.lane_cols, kWavesPerBlock};
+    e.kernel = reinterpret_cast<const void*>(
+        &lup_det_kernel<N, sh.lanes, sh.lane_cols, kPolicy, kArith, kWavesPerBlock, 1>);
+  }
 }
 
 template <int N>
 void register_one() {
   if constexpr (N % LUFQHE_SHARDS == LUFQHE_SHARD) {
-    constexpr int L = default_group_lanes<N>();
-    constexpr int CG = default_lane_cols<N>();
-    KernelEntry& e = g_kernels[LUFQHE_VARIANT][N];
-    e.launch = &launch<N>;
-    e.shape = KernelShape{L, CG, kWavesPerBlock};
-    e.kernel = reinterpret_cast<const void*>(
-        &lup_det_kernel<N, L, CG, kPolicy, kArith, kWavesPerBlock, 1>);
+    register_shape<N, 0>();
+    register_shape<N, 1>();
+    register_shape<N, 2>();
   }
 }
 

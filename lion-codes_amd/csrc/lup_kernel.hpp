@@ -471,21 +471,38 @@ __device__ __forceinline__ void write_result(const DetOut& out, int64_t m,
 //   uniform addressing : A + i*stride + r*ld            (offs == nullptr)
 //   indexed addressing : A + offs[i] + r*N              (mixed-size buckets)
 // optionally with its rows gathered through row_perm (row r of matrix i is
-// source row row_perm[i*N + r]), and its results written to output slot
-// out_idx[i] (or i).  One task per wave: a loop over tasks would keep every
+// source row p[r], p = row_perm + perm_offs[i] or row_perm + i*N), and its
+// results written to output slot out_idx[i] (or i).  One task per wave: a loop over tasks would keep every
 // step's loop-invariant addresses live across iterations and double the
 // register footprint.
 // ---------------------------------------------------------------------------
+enum Source { SRC_MATRIX = 0, SRC_FQHE = 1 };
+
 struct BatchArgs {
-  const double2* A;
+  const double2* A;  // SRC_MATRIX: the matrices; SRC_FQHE: particle positions
   int64_t count;
   int64_t ld, stride;
   const int64_t* offs;
   const int64_t* out_idx;
   const uint8_t* row_perm;
+  const int64_t* perm_offs;  // row perm of matrix i at row_perm + perm_offs[i] (else + i*N)
   int32_t piv_stride;
+  int32_t src;
   DetOut out;
 };
+
+// FQHE wavefunction matrix row (LU-fqhe's physics; the reference builds its
+// matrices on the host or in the disabled PERM builder,
+// LU-fqhe/src/mkMatrixElements.cu:169-358): A_ij = z_i^j * exp(-|z_i|^2/4).
+// Powers are a sequential product z^j = z^(j-1) * z from z^0 = 1 and every
+// entry is computed the same way whatever the launch shape, so a matrix
+// built in a kernel's load phase equals the one fqhe_build_kernel writes.
+__device__ __forceinline__ double fqhe_gauss(double2 z) {
+  return exp(-0.25 * fma(z.x, z.x, z.y * z.y));
+}
+__device__ __forceinline__ double2 cmul_ref(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
+}
 
 template <int N, int L, int CG, int POLICY, int ARITH, int WPB, int OCC>
 __global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) {
@@ -511,12 +528,29 @@ __global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) 
     const double2* __restrict__ base = nullptr;
     if (mat_ok) base = args.offs ? args.A + args.offs[i] : args.A + i * args.stride;
     const int64_t ld = args.offs ? (int64_t)N : args.ld;
+    const uint8_t* __restrict__ perm = nullptr;
+    if (mat_ok && args.row_perm)
+      perm = args.row_perm + (args.perm_offs ? args.perm_offs[i] : i * N);
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       const int row = rb + t * RG;
-      if (mat_ok && row < N) {
+      if (args.src == SRC_FQHE && mat_ok && row < N) {
+        // positions: matrix i's z at A + offs[i] (or i*stride), one per row
         int src = row;
-        if (args.row_perm) src = min((int)args.row_perm[i * N + row], N - 1);
+        if (perm) src = min((int)perm[row], N - 1);
+        const double2 z = base[src];
+        const double g = fqhe_gauss(z);
+        double2 pw = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int s = 0; s < C; ++s) a[t][s] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          if (j % CG == cb) a[t][j / CG] = make_double2(pw.x * g, pw.y * g);
+          pw = cmul_ref(pw, z);
+        }
+      } else if (mat_ok && row < N) {
+        int src = row;
+        if (perm) src = min((int)perm[row], N - 1);
         const double2* __restrict__ rp = base + (int64_t)src * ld;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
@@ -553,15 +587,26 @@ __global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) 
   }
 }
 
-// Default launch shape per n: lanes per matrix and lane-columns.
-template <int N>
-constexpr int default_group_lanes() {
-  return N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+// Candidate launch shapes per n, as (lanes per matrix L, lane-columns CG).
+// Every candidate is compiled; the autotuner picks one per n on the device
+// and candidate 0 is the default before tuning.  The lists come from shape
+// sweeps on MI355X: small n wants many matrices per wave (even two rows per
+// lane), mid n a split of each row over lane-columns, n > 32 one matrix per
+// wave.
+struct ShapeLC {
+  int lanes;
+  int lane_cols;
+};
+constexpr int kMaxShapes = 3;
+constexpr ShapeLC candidate_shape(int n, int idx) {
+  if (n <= 4) return idx == 0 ? ShapeLC{4, 1} : idx == 1 ? ShapeLC{8, 1} : ShapeLC{0, 0};
+  if (n <= 8) return idx == 0 ? ShapeLC{4, 1} : idx == 1 ? ShapeLC{8, 1} : ShapeLC{0, 0};
+  if (n <= 16) return idx == 0 ? ShapeLC{16, 1} : idx == 1 ? ShapeLC{8, 1} : ShapeLC{0, 0};
+  if (n <= 24) return idx == 0 ? ShapeLC{32, 4} : idx == 1 ? ShapeLC{32, 1} : ShapeLC{32, 2};
+  if (n <= 32) return idx == 0 ? ShapeLC{32, 2} : idx == 1 ? ShapeLC{32, 4} : ShapeLC{32, 1};
+  if (n <= 40) return idx == 0 ? ShapeLC{64, 4} : idx == 1 ? ShapeLC{64, 1} : ShapeLC{0, 0};
+  if (n <= 48) return idx == 0 ? ShapeLC{64, 4} : idx == 1 ? ShapeLC{64, 2} : ShapeLC{0, 0};
+  return idx == 0 ? ShapeLC{64, 2} : idx == 1 ? ShapeLC{64, 4} : ShapeLC{0, 0};
 }
-template <int N>
-constexpr int default_lane_cols() {
-  return (N > 16 && N <= 40) ? 2 : 1;
-}
-
 
 }  // namespace lufqhe

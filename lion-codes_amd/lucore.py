@@ -18,6 +18,8 @@ import torch  # imported before the library so both share one HIP runtime
 
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "liblufqhe.so"
+# Launch shapes chosen by the autotuner on the target device (see autotune.py).
+SHAPE_TABLE = _PKG / "shapes_gfx950.json"
 
 POLICIES = {"crout_ref": 0, "abs": 1}
 ARITHS = {"fused": 0, "ref": 1}
@@ -72,8 +74,79 @@ def load() -> ctypes.CDLL:
     lib.lufqhe_global_reduction.argtypes = [_vp, _i64, _i64, _vp, _vp, _vp]
     lib.lufqhe_pair_sum.restype = _i32
     lib.lufqhe_pair_sum.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp, _vp]
+    lib.lufqhe_shape_count.restype = _i32
+    lib.lufqhe_shape_count.argtypes = [_i32]
+    lib.lufqhe_shape_info.restype = _i32
+    lib.lufqhe_shape_info.argtypes = [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]
+    lib.lufqhe_set_shape.restype = _i32
+    lib.lufqhe_set_shape.argtypes = [_i32, _i32, _i32, _i32]
+    lib.lufqhe_get_shape.restype = _i32
+    lib.lufqhe_get_shape.argtypes = [_i32, _i32, _i32]
+    lib.lufqhe_mixed_plan_create.restype = _i32
+    lib.lufqhe_mixed_plan_create.argtypes = [_vp, _vp, _i64, _i32, _i32, ctypes.POINTER(_vp)]
+    lib.lufqhe_mixed_plan_destroy.restype = _i32
+    lib.lufqhe_mixed_plan_destroy.argtypes = [_vp]
+    lib.lufqhe_mixed_plan_buckets.restype = _i32
+    lib.lufqhe_mixed_plan_buckets.argtypes = [_vp, _vp, _vp, _vp, _i32]
+    lib.lufqhe_mixed_execute.restype = _i32
+    lib.lufqhe_mixed_execute.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _vp]
+    lib.lufqhe_fqhe_det.restype = _i32
+    lib.lufqhe_fqhe_det.argtypes = [_vp, _i64, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _i32, _i32, _vp]
+    lib.lufqhe_server_create.restype = _i32
+    lib.lufqhe_server_create.argtypes = [_i32, _vp, _i32, ctypes.c_uint64, _i32,
+                                         ctypes.POINTER(_vp)]
+    lib.lufqhe_server_destroy.argtypes = [_vp]
+    lib.lufqhe_server_reset.restype = _i32
+    lib.lufqhe_server_reset.argtypes = [_vp]
+    lib.lufqhe_server_walker.restype = _i32
+    lib.lufqhe_server_walker.argtypes = [_vp, _i32, ctypes.POINTER(_i32), _vp]
+    lib.lufqhe_server_perms.restype = _i32
+    lib.lufqhe_server_perms.argtypes = [_vp, _i32, _vp]
+    lib.lufqhe_server_run.restype = _i32
+    lib.lufqhe_server_run.argtypes = [_vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeStats),
+                                      _vp]
+    lib.lufqhe_fqhe_build.restype = _i32
+    lib.lufqhe_fqhe_build.argtypes = [_vp, _i64, _i32, _i64, _vp, _vp, _vp]
     _lib = lib
+    apply_shape_table()
     return lib
+
+
+def apply_shape_table(path: Path | None = None) -> int:
+    """Select the tuned launch shape of every (policy, n) listed in the table
+    written by autotune.py; returns the number of entries applied."""
+    p = Path(path) if path is not None else SHAPE_TABLE
+    if not p.exists():
+        return 0
+    import json
+    table = json.loads(p.read_text())
+    applied = 0
+    for e in table.get("entries", []):
+        if set_shape(e["policy"], "fused", int(e["n"]), int(e["shape"])):
+            applied += 1
+    return applied
+
+
+def shape_count(n: int) -> int:
+    return load().lufqhe_shape_count(n)
+
+
+def shape_info(n: int, shape: int) -> tuple[int, int]:
+    """(lanes per matrix, lane-columns) of candidate `shape` for size n."""
+    lanes, cols = _i32(), _i32()
+    st = load().lufqhe_shape_info(n, shape, ctypes.byref(lanes), ctypes.byref(cols))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_shape_info", st)
+    return lanes.value, cols.value
+
+
+def set_shape(policy: str, arith: str, n: int, shape: int) -> bool:
+    return _lib.lufqhe_set_shape(POLICIES[policy], ARITHS[arith], n, shape) == SUCCESS
+
+
+def get_shape(policy: str, arith: str, n: int) -> int:
+    return load().lufqhe_get_shape(POLICIES[policy], ARITHS[arith], n)
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -168,6 +241,210 @@ def pair_sum(res: DetResult, workspace: torch.Tensor | None = None,
     if st != SUCCESS:
         raise LufqheError("lufqhe_pair_sum", st)
     return mant, exp
+
+
+def _check_fqhe_inputs(z: torch.Tensor, row_perm: torch.Tensor | None):
+    if z.dtype != torch.complex128 or not z.is_cuda or z.dim() != 2 or z.stride(1) != 1:
+        raise ValueError("z must be a complex128 CUDA tensor [count or 1, n], unit stride")
+    if row_perm is not None and (row_perm.dtype != torch.uint8 or not row_perm.is_cuda
+                                 or not row_perm.is_contiguous()):
+        raise ValueError("row_perm must be a contiguous uint8 CUDA tensor [count, n]")
+
+
+def fqhe_det(z: torch.Tensor, row_perm: torch.Tensor | None = None, count: int | None = None,
+             policy: str = "crout_ref", arith: str = "fused", pivots: bool = False,
+             out: DetResult | None = None,
+             stream: torch.cuda.Stream | None = None) -> DetResult:
+    """Determinants of FQHE matrices A_ij = z_i^j exp(-|z_i|^2/4) built on the
+    device from positions ``z`` ([count, n] complex128, or [1, n] shared by
+    all ``count`` matrices), rows optionally permuted by ``row_perm`` [count, n]."""
+    _check_fqhe_inputs(z, row_perm)
+    rows, n = z.shape
+    z_stride = z.stride(0) if rows > 1 else 0
+    count = rows if count is None else count
+    if rows > 1 and count != rows:
+        raise ValueError("count must equal z.shape[0] unless z holds one configuration")
+    if row_perm is not None and tuple(row_perm.shape) != (count, n):
+        raise ValueError("row_perm must have shape [count, n]")
+    res = out if out is not None else alloc_result(count, n, z.device, pivots)
+    st = load().lufqhe_fqhe_det(_ptr(z), count, n, z_stride, _ptr(row_perm), _ptr(res.mant),
+                                _ptr(res.exp), _ptr(res.logabs), _ptr(res.phase),
+                                _ptr(res.info), _ptr(res.piv), POLICIES[policy], ARITHS[arith],
+                                _stream_handle(stream))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_fqhe_det", st)
+    return res
+
+
+def fqhe_build(z: torch.Tensor, row_perm: torch.Tensor | None = None, count: int | None = None,
+               stream: torch.cuda.Stream | None = None) -> torch.Tensor:
+    """The FQHE matrices fqhe_det builds, written out ([count, n, n] complex128)."""
+    _check_fqhe_inputs(z, row_perm)
+    rows, n = z.shape
+    z_stride = z.stride(0) if rows > 1 else 0
+    count = rows if count is None else count
+    if rows > 1 and count != rows:
+        raise ValueError("count must equal z.shape[0] unless z holds one configuration")
+    if row_perm is not None and tuple(row_perm.shape) != (count, n):
+        raise ValueError("row_perm must have shape [count, n]")
+    A = torch.empty(count, n, n, dtype=torch.complex128, device=z.device)
+    st = load().lufqhe_fqhe_build(_ptr(z), count, n, z_stride, _ptr(row_perm), _ptr(A),
+                                  _stream_handle(stream))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_fqhe_build", st)
+    return A
+
+
+class ServeConfig(ctypes.Structure):
+    """lufqhe_serve_config."""
+    _fields_ = [("rate", ctypes.c_double), ("requests", _i64), ("k_min", _i32), ("k_max", _i32),
+                ("seed", ctypes.c_uint64), ("max_batch_requests", _i32), ("max_inflight", _i32),
+                ("max_batch_dets", _i64), ("hbm_budget", _i64)]
+
+
+class ServeStats(ctypes.Structure):
+    """lufqhe_serve_stats."""
+    _fields_ = [("requests", _i64), ("dets", _i64), ("batches", _i64), ("wall_s", ctypes.c_double),
+                ("dets_per_s", ctypes.c_double), ("requests_per_s", ctypes.c_double),
+                ("offered_rate", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double),
+                ("mean_s", ctypes.c_double), ("mean_batch_dets", ctypes.c_double),
+                ("budget_waits", _i64), ("peak_inflight_bytes", _i64), ("resident_bytes", _i64)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# numpy view of lufqhe_serve_record
+SERVE_RECORD_DTYPE = [("walker", "<i4"), ("particle", "<i4"), ("k", "<i4"), ("perm_start", "<i4"),
+                      ("accepted", "<i4"), ("sum_exp", "<i4"), ("delta", "<c16"),
+                      ("sum_mant", "<c16"), ("latency_s", "<f8")]
+
+
+class Server:
+    """Determinant server for Monte Carlo walkers (lufqhe_server_*): walker
+    configurations and permutation tables live on the device; run() serves a
+    Poisson stream of Monte Carlo steps with continuous batching."""
+
+    def __init__(self, walker_n, perms_per_walker: int = 2000, seed: int = 0,
+                 policy: str = "crout_ref"):
+        import numpy as np
+        self.walker_n = np.ascontiguousarray(walker_n, dtype=np.int32)
+        self.walkers = len(self.walker_n)
+        self.perms_per_walker = perms_per_walker
+        self._h = _vp()
+        st = load().lufqhe_server_create(self.walkers, self.walker_n.ctypes.data, perms_per_walker,
+                                         seed, POLICIES[policy], ctypes.byref(self._h))
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_server_create", st)
+
+    def run(self, requests: int, rate: float = 0.0, k_min: int = 1, k_max: int = 2000,
+            seed: int = 1, max_batch_requests: int = 256, max_inflight: int = 4,
+            max_batch_dets: int = 1 << 20, hbm_budget: int = 200 << 30,
+            records: bool = True):
+        """Serve `requests` Monte Carlo steps; returns (stats dict, records
+        structured array or None)."""
+        import numpy as np
+        cfg = ServeConfig(rate, requests, k_min, k_max, seed, max_batch_requests, max_inflight,
+                          max_batch_dets, hbm_budget)
+        stats = ServeStats()
+        rec = np.zeros(requests, dtype=SERVE_RECORD_DTYPE) if records else None
+        st = _lib.lufqhe_server_run(self._h, ctypes.byref(cfg), ctypes.byref(stats),
+                                    None if rec is None else rec.ctypes.data)
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_server_run", st)
+        return stats.as_dict(), rec
+
+    def reset(self) -> None:
+        st = _lib.lufqhe_server_reset(self._h)
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_server_reset", st)
+
+    def walker(self, w: int):
+        """Walker w's current configuration (complex128 [n])."""
+        import numpy as np
+        n = _i32()
+        z = np.zeros(64, np.complex128)
+        st = _lib.lufqhe_server_walker(self._h, w, ctypes.byref(n), z.ctypes.data)
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_server_walker", st)
+        return z[:n.value].copy()
+
+    def perms(self, w: int):
+        """Walker w's permutation table (uint8 [perms_per_walker, n])."""
+        import numpy as np
+        t = np.zeros((self.perms_per_walker, 64), np.uint8)
+        st = _lib.lufqhe_server_perms(self._h, w, t.ctypes.data)
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_server_perms", st)
+        return t[:, :int(self.walker_n[w])].copy()
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lufqhe_server_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MIXED_MODES = {"graph": 0, "streams": 1, "plain": 2}
+SOURCES = {"matrix": 0, "fqhe": 1}
+
+
+class MixedPlan:
+    """A mixed-size batch plan (lufqhe_mixed_plan_*): matrix m has side ns[m]
+    and starts at complex element offs[m] of one flat buffer."""
+
+    def __init__(self, ns, offs, policy: str = "crout_ref", nstreams: int = 4):
+        import numpy as np
+        self.ns = np.ascontiguousarray(ns, dtype=np.int32)
+        self.offs = np.ascontiguousarray(offs, dtype=np.int64)
+        self.count = len(self.ns)
+        self._h = _vp()
+        st = load().lufqhe_mixed_plan_create(self.ns.ctypes.data, self.offs.ctypes.data,
+                                             self.count, POLICIES[policy], nstreams,
+                                             ctypes.byref(self._h))
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_mixed_plan_create", st)
+
+    def buckets(self):
+        """[(n, count, stream)] in launch order."""
+        import numpy as np
+        nb = _lib.lufqhe_mixed_plan_buckets(self._h, None, None, None, 0)
+        n = np.empty(nb, np.int32)
+        c = np.empty(nb, np.int64)
+        s = np.empty(nb, np.int32)
+        _lib.lufqhe_mixed_plan_buckets(self._h, n.ctypes.data, c.ctypes.data, s.ctypes.data, nb)
+        return list(zip(n.tolist(), c.tolist(), s.tolist()))
+
+    def execute(self, flat: torch.Tensor, out: DetResult, mode: str = "graph",
+                stream: torch.cuda.Stream | None = None, source: str = "matrix") -> DetResult:
+        """source "matrix": matrix m at flat[offs[m]:]; "fqhe": its ns[m]
+        particle positions there, the FQHE matrix built on the device."""
+        if flat.dtype != torch.complex128 or not flat.is_cuda:
+            raise ValueError("flat must be a complex128 CUDA tensor")
+        st = _lib.lufqhe_mixed_execute(self._h, _ptr(flat), _ptr(out.mant), _ptr(out.exp),
+                                       _ptr(out.logabs), _ptr(out.phase), _ptr(out.info),
+                                       SOURCES[source], MIXED_MODES[mode],
+                                       _stream_handle(stream))
+        if st != SUCCESS:
+            raise LufqheError("lufqhe_mixed_execute", st)
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lufqhe_mixed_plan_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def logabs_phase_from(mant_re, mant_im, exp):

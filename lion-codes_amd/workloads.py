@@ -78,6 +78,35 @@ def fqhe_batch(count: int, n_lo: int, n_hi: int, seed: int):
     return flat, ns, offs, zs
 
 
+def fqhe_configs(count: int, n_lo: int, n_hi: int, seed: int):
+    """Packed particle configurations, n ~ U{n_lo..n_hi}:
+    (z_flat complex128, ns int32, offs int64) with matrix m's positions at
+    z_flat[offs[m] : offs[m] + ns[m]] -- the input of the on-device builder."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_lo, n_hi + 1, size=count).astype(np.int32)
+    offs = np.zeros(count, np.int64)
+    offs[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+    total = int(offs[-1] + ns[-1]) if count else 0
+    R = np.sqrt(2.0 * np.repeat(ns, ns).astype(np.float64))
+    u = rng.random(total)
+    v = rng.random(total)
+    z = R * np.sqrt(u) * np.exp(2j * np.pi * v)
+    return z, ns, offs
+
+
+def fqhe_matrix_seq(z: np.ndarray) -> np.ndarray:
+    """fqhe_matrix with powers as sequential products z^j = z^(j-1) * z (the
+    device builder's order)."""
+    n = len(z)
+    A = np.empty((n, n), np.complex128)
+    g = np.exp(-0.25 * (z.real * z.real + z.imag * z.imag))
+    pw = np.ones(n, np.complex128)
+    for j in range(n):
+        A[:, j] = pw * g
+        pw = pw * z
+    return A
+
+
 # --------------------------------------------------------------------------- mixed
 def mixed_sizes(count: int, n_lo: int, n_hi: int, seed: int):
     rng = np.random.default_rng(seed)
@@ -101,7 +130,7 @@ def mixed_batch(count: int, seed: int, n_lo: int = 8, n_hi: int = 64,
     flat = torch.complex(torch.randn(total, generator=g, dtype=torch.float64, device=device),
                          torch.randn(total, generator=g, dtype=torch.float64, device=device))
     rng = np.random.default_rng(seed + 2)
-    sing = rng.random(count) < near_singular
+    sing = (rng.random(count) < near_singular) & (ns >= 3)  # needs 3 distinct rows
     idx = np.nonzero(sing)[0]
     for m in idx:
         n = int(ns[m])

@@ -69,3 +69,34 @@ def test_epilogue_argument_validation(lc):
     assert lib.lufqhe_pair_sum(None, None, 4, None, None, None, None) == 1
     assert lib.lufqhe_det_inplace_workspace_bytes(10) == 200
     assert lib.lufqhe_pair_sum_workspace_bytes() > 0
+
+
+def test_shape_table_api(lc):
+    lib = lc.load()
+    for n in range(1, 65):
+        c = lib.lufqhe_shape_count(n)
+        assert 1 <= c <= 3
+        for sh in range(c):
+            lanes, cols = lc.shape_info(n, sh)
+            assert lanes in (4, 8, 16, 32, 64) and cols in (1, 2, 4) and lanes % cols == 0
+    assert lib.lufqhe_shape_count(0) == 0 and lib.lufqhe_shape_count(65) == 0
+    assert lib.lufqhe_set_shape(0, 0, 32, 7) == 1          # shape out of range
+    assert lib.lufqhe_set_shape(0, 1, 32, 1) == 4          # reference arithmetic: shape 0 only
+    assert lib.lufqhe_set_shape(1, 1, 32, 0) == 2          # ABS + reference arithmetic
+    assert lib.lufqhe_get_shape(0, 0, 0) == -1
+    old = lc.get_shape("crout_ref", "fused", 32)
+    assert lc.set_shape("crout_ref", "fused", 32, 2) and lc.get_shape("crout_ref", "fused", 32) == 2
+    assert lc.set_shape("crout_ref", "fused", 32, old)
+
+
+def test_fqhe_argument_validation(lc):
+    lib = lc.load()
+    args = (None, None, None, None, None, None, 0, 0, None)
+    assert lib.lufqhe_fqhe_det(None, 0, 8, 8, None, *args) == 0     # empty batch
+    assert lib.lufqhe_fqhe_det(None, 4, 8, 5, None, *args) == 1     # 0 < z_stride < n
+    assert lib.lufqhe_fqhe_det(None, 4, 65, 65, None, *args) == 1   # n > 64
+    assert lib.lufqhe_fqhe_det(None, 4, 8, 8, None, *args) == 1     # null positions
+    assert lib.lufqhe_fqhe_det(None, 4, 8, 0, None, None, None, None, None, None, None,
+                               1, 1, None) == 2                      # ABS + reference arithmetic
+    assert lib.lufqhe_fqhe_build(None, 4, 8, 8, None, None, None) == 1
+    assert lib.lufqhe_fqhe_build(None, 0, 8, 8, None, None, None) == 0

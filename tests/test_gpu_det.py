@@ -159,3 +159,19 @@ def test_full_size_uniform_configs(lc, orc, wl, dev, n, count):
     e = np.abs(np.expm1((r.logabs[sample.to(dev)].cpu().numpy() - t["logabs"])
                         + 1j * np.angle(np.exp(1j * (r.phase[sample.to(dev)].cpu().numpy() - t["phase"])))))
     assert np.median(e) < 1e-13 and np.quantile(e, 0.99) < 1e-11
+
+
+@pytest.mark.parametrize("n", ALL_N)
+def test_every_launch_shape_bit_exact(lc, orc, wl, dev, n):
+    """Every compiled candidate shape (the autotuner may pick any) gives the
+    oracle's determinants and pivots bit for bit."""
+    A = wl.gaussian_np(45, n, seed=3000 + n)
+    for policy in ("crout_ref", "abs"):
+        o = orc.mirror(A, policy, "fused")
+        keep = lc.get_shape(policy, "fused", n)
+        try:
+            for shape in range(lc.shape_count(n)):
+                assert lc.set_shape(policy, "fused", n, shape)
+                assert_same(run(lc, A, dev, policy, "fused"), o, n, (policy, shape))
+        finally:
+            lc.set_shape(policy, "fused", n, keep)
