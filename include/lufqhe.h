@@ -43,6 +43,7 @@ extern "C" {
 #define LUFQHE_ERROR_HIP 3
 #define LUFQHE_ERROR_NOT_BUILT 4
 #define LUFQHE_ERROR_OUT_OF_BUDGET 5
+#define LUFQHE_ERROR_IO 6
 
 /* Pivot policies.  CROUT_REF is the reference's rule: argmax over remaining
  * rows of |Re u_ik| / s_i, s_i = max_j |Re a_ij| of the original row, ties to
@@ -199,6 +200,33 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan *plan, const void *d_A,
                              void *d_mant, int32_t *d_exp, double *d_logabs,
                              double *d_phase, int32_t *d_info, int32_t source,
                              int32_t mode, hipStream_t stream);
+
+/* ---- the reference's matrix input file ---------------------------------
+ * LU mode reads `count` matrices of side n from one raw file
+ * (LU-fqhe/src/PermHandler.cpp:91-170): elements (re, im) interleaved as
+ * float32 (LUFQHE_FILE_COMPLEX64; float64 for LUFQHE_FILE_COMPLEX128, the
+ * __DPREC__ build), row major, unpadded, no header.  Rank r of `nodes`
+ * reads one run of scalars whose bounds the reference computes in float
+ * arithmetic: first = (int)((float)(r*total)/nodes), last =
+ * (int)((float)((r+1)*total)/nodes) - 1, total = 2*count*n*n (a run can
+ * start or end inside a matrix, as in the reference).
+ *
+ * lufqhe_file_chunk: that run (first scalar, number of scalars).
+ * lufqhe_read_matrices: reads the run into d_out as complex128 scalars
+ * (capacity = room in d_out, in float64 scalars); complex64 data crosses
+ * the bus as float32 and is widened on the device.  A run that reaches past
+ * the end of the file is cut there; *read_out gets the scalars read.
+ * Synchronous with respect to the host (returns after the data is on the
+ * device); allocates two 32 MiB pinned staging buffers for the call. */
+#define LUFQHE_FILE_COMPLEX64 0
+#define LUFQHE_FILE_COMPLEX128 1
+
+int32_t lufqhe_file_chunk(int64_t count, int32_t n, int32_t rank,
+                          int32_t nodes, int64_t *first, int64_t *length);
+int32_t lufqhe_read_matrices(const char *path, int32_t format, int64_t count,
+                             int32_t n, int32_t rank, int32_t nodes,
+                             void *d_out, int64_t capacity, int64_t *read_out,
+                             hipStream_t stream);
 
 /* ---- serving: determinants for Monte Carlo walkers ----------------------
  * The reference's Monte Carlo loop (LU-fqhe/src/MonteCarloLoop.cu:280-400)

@@ -14,12 +14,14 @@ from .lucore import (  # noqa: F401
     det_batched,
     apply_shape_table,
     det_inplace,
+    file_chunk,
     fqhe_build,
     fqhe_det,
     get_shape,
     global_reduction,
     load,
     pair_sum,
+    read_matrices,
     set_shape,
     shape_count,
     shape_info,

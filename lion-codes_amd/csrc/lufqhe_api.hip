@@ -89,6 +89,7 @@ const char* lufqhe_status_string(int32_t status) {
     case LUFQHE_ERROR_HIP: return "HIP runtime error";
     case LUFQHE_ERROR_NOT_BUILT: return "kernel variant not built";
     case LUFQHE_ERROR_OUT_OF_BUDGET: return "memory budget exceeded";
+    case LUFQHE_ERROR_IO: return "file cannot be read";
     default: return "unknown status";
   }
 }

@@ -213,6 +213,203 @@ struct GroupResult {
   int info;
 };
 
+// Pivot search of one step as a sequence of stages, so that it can be
+// spread over the update of the previous step (see update_and_search).
+//   stage 0                 criterion of this lane's candidates, best first
+//                           (ties: the larger position)
+//   stages 1 .. LOG         butterfly max of the criterion over the group
+//   stage LOG+1             candidates equal to the max keep their position
+//   stages LOG+2 .. 2LOG+1  butterfly max of those positions
+// The result, the position of the exact arg-max with ties to the larger
+// position, is in pm after the last stage.
+struct SearchState {
+  double cl;  // this lane's best criterion
+  int pl;     // its position
+  double cm;  // running group max of the criterion
+  int pm;     // running group max of the candidate positions
+};
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+template <int L>
+constexpr int search_stages() { return 2 * ilog2(L) + 2; }
+
+// One butterfly level i (0-based) of the group max, for doubles and ints.
+template <int L>
+__device__ __forceinline__ double max_level_d(int i, double v) {
+  switch (i) {
+    case 0: return vmax_f64(v, dpp_d<kQuadXor1>(v));
+    case 1: return vmax_f64(v, dpp_d<kQuadXor2>(v));
+    case 2: return vmax_f64(v, dpp_d<kHalfMirror>(v));
+    case 3: return vmax_f64(v, dpp_d<kMirror>(v));
+    case 4: {
+      const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+      const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+      const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+      return vmax_f64(__hiloint2double(sh[0], sl[0]), __hiloint2double(sh[1], sl[1]));
+    }
+    default: {
+      const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+      const auto sl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+      const auto sh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+      return vmax_f64(__hiloint2double(sh[0], sl[0]), __hiloint2double(sh[1], sl[1]));
+    }
+  }
+}
+template <int L>
+__device__ __forceinline__ int max_level_i(int i, int v) {
+  switch (i) {
+    case 0: return max(v, dpp_i<kQuadXor1>(v));
+    case 1: return max(v, dpp_i<kQuadXor2>(v));
+    case 2: return max(v, dpp_i<kHalfMirror>(v));
+    case 3: return max(v, dpp_i<kMirror>(v));
+    case 4: {
+      const auto s = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+      return max((int)s[0], (int)s[1]);
+    }
+    default: {
+      const auto s = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+      return max((int)s[0], (int)s[1]);
+    }
+  }
+}
+
+template <int L, int CG, int POLICY, int ARITH, int R, int C>
+__device__ __forceinline__ void search_stage(int j, SearchState& st, const double2 (&a)[R][C],
+                                             const int (&pos)[R], const bool (&act)[R],
+                                             const double (&sc)[R], const double (&rs)[R],
+                                             int cb, int k) {
+  constexpr int LOG = ilog2(L);
+  if (j == 0) {
+    const int ck = k % CG, sk = k / CG;
+    double cbest = -1.0;
+    int pbest = -1;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const double2 v = a[t][sk];
+      double c;
+      if constexpr (POLICY == POLICY_CROUT_REF) {
+        if constexpr (ARITH == ARITH_FUSED)
+          c = fabs(v.x) * rs[t];
+        else
+          c = fabs(v.x) / sc[t];
+      } else {
+        if constexpr (ARITH == ARITH_FUSED)
+          c = fma(v.x, v.x, v.y * v.y);
+        else
+          c = v.x * v.x + v.y * v.y;
+      }
+      c = fmin(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
+      c = (act[t] && cb == ck) ? c : -1.0;
+      if (t == 0) {
+        cbest = c;
+        pbest = pos[0];
+      } else {
+        const bool better = (c > cbest) || (c == cbest && pos[t] > pbest);
+        cbest = better ? c : cbest;
+        pbest = better ? pos[t] : pbest;
+      }
+    }
+    st.cl = cbest;
+    st.pl = pbest;
+    st.cm = cbest;
+  } else if (j <= LOG) {
+    st.cm = max_level_d<L>(j - 1, st.cm);
+  } else if (j == LOG + 1) {
+    st.pm = (st.cl == st.cm) ? st.pl : -1;
+  } else {
+    st.pm = max_level_i<L>(j - LOG - 2, st.pm);
+  }
+}
+
+__device__ __forceinline__ void pin(SearchState& st) {
+  asm volatile("" : "+v"(st.cl), "+v"(st.pl), "+v"(st.cm), "+v"(st.pm)::"memory");
+}
+
+// a[t][s] -= l_t * u_s for one column slot s (4 FMAs per complex element).
+template <int ARITH, int R, int C>
+__device__ __forceinline__ void update_slot(double2 (&a)[R][C], const double (&lx)[R],
+                                            const double (&ly)[R], int s, double2 u) {
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    double2& e = a[t][s];
+    if constexpr (ARITH == ARITH_FUSED) {
+      double x = fma(-lx[t], u.x, e.x);
+      x = fma(ly[t], u.y, x);
+      double y = fma(-lx[t], u.y, e.y);
+      y = fma(-ly[t], u.x, y);
+      e.x = x;
+      e.y = y;
+    } else {
+      const double tx = lx[t] * u.x - ly[t] * u.y;
+      const double ty = ly[t] * u.x + lx[t] * u.y;
+      e.x = e.x - tx;
+      e.y = e.y - ty;
+    }
+  }
+}
+
+template <int R, int C>
+__device__ __forceinline__ void pin_slot(double2 (&a)[R][C], int s) {
+#pragma unroll
+  for (int t = 0; t < R; ++t) asm volatile("" : "+v"(a[t][s].x), "+v"(a[t][s].y)::"memory");
+}
+
+// The rank-1 update of step k (column slots s0 .. C-1, s0 = (k+1)/CG) with
+// step k+1's pivot search woven into it.  Slot s0, which holds column k+1,
+// is updated first; then the remaining slots stream from LDS in double-
+// buffered chunks of CH slots, and after each chunk's FMAs come the next few
+// search stages.  A pin after each chunk holds both the chunk's results and
+// the search state, so the search's dependent chain of cross-lane moves
+// sits between independent FMAs instead of stalling the wave, and the
+// compiler cannot hoist every read of the step (which would hold 4*(n-k)
+// extra registers).  Each element sees the same operations in the same
+// order as in a plain step-by-step elimination.
+template <int L, int CG, int POLICY, int ARITH, int R, int C>
+__device__ __forceinline__ int update_and_search(
+    double2 (&a)[R][C], const double (&lx)[R], const double (&ly)[R],
+    const double2* __restrict__ bc, int cb, int k, const int (&pos)[R], const bool (&act)[R],
+    const double (&sc)[R], const double (&rs)[R]) {
+  constexpr int NS = search_stages<L>();
+  constexpr int CH = 4;
+  const int s0 = (k + 1) / CG;
+  update_slot<ARITH>(a, lx, ly, s0, bc[cb + s0 * CG]);
+  pin_slot(a, s0);
+
+  SearchState st{-1.0, -1, -1.0, -1};
+  const int s1 = s0 + 1;
+  const int nch = (C - s1 + CH - 1) / CH;
+  if (nch == 0) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j) search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, k + 1);
+    return st.pm;
+  }
+  double2 ub[2][CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+    if (s1 + c < C) ub[0][c] = bc[cb + (s1 + c) * CG];
+#pragma unroll
+  for (int ci = 0; ci < nch; ++ci) {
+    const int c0 = s1 + ci * CH;
+    if (ci + 1 < nch) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+        if (c0 + CH + c < C) ub[(ci + 1) & 1][c] = bc[cb + (c0 + CH + c) * CG];
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (c0 + c < C) update_slot<ARITH>(a, lx, ly, c0 + c, ub[ci & 1][c]);
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+      if (j * nch / NS == ci) search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, k + 1);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (c0 + c < C) pin_slot(a, c0 + c);
+    pin(st);
+  }
+  return st.pm;
+}
+
 // Factorisation of one matrix held by a group as a[R][C] (see Tile).  Rows
 // never move; each row slot carries its reference position.  piv_rec[q]
 // receives the pivot position chosen at step gl + q*L.
@@ -261,51 +458,26 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
 
   double dx = 1.0, dy = 0.0, sg = 1.0;
   int de = 0, info = 0;
+  int ppos;
+  {
+    SearchState st{-1.0, -1, -1.0, -1};
+#pragma unroll
+    for (int j = 0; j < search_stages<L>(); ++j)
+      search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, 0);
+    ppos = st.pm;
+  }
 
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int ck = k % CG, sk = k / CG;
-
-    // 1. criterion of this lane's candidates, best first (ties: larger pos)
-    double cbest = -1.0;
-    int pbest = -1;
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const double2 v = a[t][sk];
-      double c;
-      if constexpr (POLICY == POLICY_CROUT_REF) {
-        if constexpr (ARITH == ARITH_FUSED)
-          c = fabs(v.x) * rs[t];
-        else
-          c = fabs(v.x) / sc[t];
-      } else {
-        if constexpr (ARITH == ARITH_FUSED)
-          c = fma(v.x, v.x, v.y * v.y);
-        else
-          c = v.x * v.x + v.y * v.y;
-      }
-      c = fmin(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
-      c = (act[t] && cb == ck) ? c : -1.0;
-      if (t == 0) {
-        cbest = c;
-        pbest = pos[0];
-      } else {
-        const bool better = (c > cbest) || (c == cbest && pos[t] > pbest);
-        cbest = better ? c : cbest;
-        pbest = better ? pos[t] : pbest;
-      }
-    }
-
-    // 2. exact arg-max over the group
-    const double cmax = group_max_d<L>(cbest);
-    const int ppos = group_max_i<L>(cbest == cmax ? pbest : -1);
     bool is_piv[R];
 #pragma unroll
     for (int t = 0; t < R; ++t) is_piv[t] = (pos[t] == ppos);
 
-    // 3. the pivot's lane-row writes the pivot row (columns >= k) to LDS:
-    // one exec-masked region per row slot, so no data selects are needed and
-    // a region whose mask is empty is branched over.
+    // The pivot's lane-row writes the pivot row (columns >= k) to LDS: one
+    // exec-masked region per row slot, so no data selects are needed and a
+    // region whose mask is empty is branched over.  The wave executes its
+    // LDS instructions in order, so the reads below see these writes.
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if (is_piv[t]) {
@@ -319,7 +491,7 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
     if (gl == k % L) piv_rec[k / L] = ppos;
     if (ppos != k) sg = -sg;
 
-    // 4. determinant product
+    // determinant product
     {
       double nx, ny;
       if constexpr (ARITH == ARITH_FUSED) {
@@ -334,7 +506,6 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
       renorm(dx, dy, de);
     }
 
-    // 5. multipliers and rank-1 update of columns > k
     double den;
     if constexpr (ARITH == ARITH_FUSED)
       den = fma(p.x, p.x, p.y * p.y);
@@ -342,6 +513,10 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
       den = p.x * p.x + p.y * p.y;
     const bool zero = (den == 0.0);
     if (zero && info == 0) info = k + 1;
+
+    // multipliers l_t = a_tk / p of this lane's rows, column k broadcast
+    // along the lane-row
+    double lx[R], ly[R];
     if (k + 1 < N) {
       double ir = 0.0, ii = 0.0;
       if constexpr (ARITH == ARITH_FUSED) {
@@ -349,14 +524,6 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
         ir = p.x * rden;
         ii = -(p.y * rden);
       }
-      const int s0 = (k + 1) / CG;  // first column slot with a column > k
-      // Every row slot is updated, with no divergent branch.  Finished rows
-      // and the pivot row are never read again after this step, so what the
-      // update leaves in them does not matter.  After a zero pivot the
-      // multipliers are exactly zero (ir = ii = 0 in fused arithmetic, an
-      // explicit zero in reference arithmetic), and fma(-0, u, e) == e for
-      // finite u up to the sign of a zero element, which no output sees.
-      double lx[R], ly[R];
 #pragma unroll
       for (int t = 0; t < R; ++t) {
         const double ax = lanerow_bcast<CG>(a[t][sk].x, ck);
@@ -369,61 +536,9 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
           ly[t] = zero ? 0.0 : (ay * p.x - ax * p.y) / den;
         }
       }
-      // Stream the pivot row through registers in chunks of CH columns,
-      // double buffered: chunk c+1 is read while chunk c is applied, and a
-      // scheduling barrier per chunk keeps the compiler from hoisting every
-      // read of the step (which would hold 4*(n-k) extra registers).
-      constexpr int CH = 4;
-      const int nch = (C - s0 + CH - 1) / CH;
-      double2 ub[2][CH];
-#pragma unroll
-      for (int c = 0; c < CH; ++c)
-        if (s0 + c < C) ub[0][c] = bc[cb + (s0 + c) * CG];
-#pragma unroll
-      for (int ci = 0; ci < nch; ++ci) {
-        const int c0 = s0 + ci * CH;
-        if (ci + 1 < nch) {
-#pragma unroll
-          for (int c = 0; c < CH; ++c)
-            if (c0 + CH + c < C) ub[(ci + 1) & 1][c] = bc[cb + (c0 + CH + c) * CG];
-        }
-#pragma unroll
-        for (int t = 0; t < R; ++t) {
-#pragma unroll
-          for (int c = 0; c < CH; ++c) {
-            if (c0 + c >= C) continue;
-            const double2 u = ub[ci & 1][c];
-            double2& e = a[t][c0 + c];
-            if constexpr (ARITH == ARITH_FUSED) {
-              double x = fma(-lx[t], u.x, e.x);
-              x = fma(ly[t], u.y, x);
-              double y = fma(-lx[t], u.y, e.y);
-              y = fma(-ly[t], u.x, y);
-              e.x = x;
-              e.y = y;
-            } else {
-              const double tx = lx[t] * u.x - ly[t] * u.y;
-              const double ty = ly[t] * u.x + lx[t] * u.y;
-              e.x = e.x - tx;
-              e.y = e.y - ty;
-            }
-          }
-        }
-        // Pin this chunk's results before the next chunk's reads: the
-        // compiler may not sink these FMAs below, nor hoist later LDS reads
-        // above, this point.
-#pragma unroll
-        for (int t = 0; t < R; ++t) {
-#pragma unroll
-          for (int c = 0; c < CH; ++c)
-            if (c0 + c < C)
-              asm volatile("" : "+v"(a[t][c0 + c].x), "+v"(a[t][c0 + c].y)::"memory");
-        }
-      }
     }
-    wave_lds_sync();
 
-    // 6. the reference's swap, as position bookkeeping
+    // the reference's swap, as position bookkeeping
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       if (pos[t] == k) pos[t] = ppos;
@@ -432,6 +547,17 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
         act[t] = false;
       }
     }
+
+    if (k + 1 < N) {
+      // Every row slot is updated, with no divergent branch.  Finished rows
+      // and the pivot row are never read again after this step, so what the
+      // update leaves in them does not matter.  After a zero pivot the
+      // multipliers are exactly zero (ir = ii = 0 in fused arithmetic, an
+      // explicit zero in reference arithmetic), and fma(-0, u, e) == e for
+      // finite u up to the sign of a zero element, which no output sees.
+      ppos = update_and_search<L, CG, POLICY, ARITH>(a, lx, ly, bc, cb, k, pos, act, sc, rs);
+    }
+    wave_lds_sync();
   }
 
   GroupResult res;

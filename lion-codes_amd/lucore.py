@@ -30,6 +30,8 @@ ERROR_UNSUPPORTED = 2
 ERROR_HIP = 3
 ERROR_NOT_BUILT = 4
 ERROR_OUT_OF_BUDGET = 5
+ERROR_IO = 6
+FILE_FORMATS = {"complex64": 0, "complex128": 1}
 
 _lib = None
 
@@ -106,6 +108,12 @@ def load() -> ctypes.CDLL:
     lib.lufqhe_server_run.restype = _i32
     lib.lufqhe_server_run.argtypes = [_vp, ctypes.POINTER(ServeConfig), ctypes.POINTER(ServeStats),
                                       _vp]
+    lib.lufqhe_file_chunk.restype = _i32
+    lib.lufqhe_file_chunk.argtypes = [_i64, _i32, _i32, _i32, ctypes.POINTER(_i64),
+                                      ctypes.POINTER(_i64)]
+    lib.lufqhe_read_matrices.restype = _i32
+    lib.lufqhe_read_matrices.argtypes = [ctypes.c_char_p, _i32, _i64, _i32, _i32, _i32, _vp, _i64,
+                                         ctypes.POINTER(_i64), _vp]
     lib.lufqhe_fqhe_build.restype = _i32
     lib.lufqhe_fqhe_build.argtypes = [_vp, _i64, _i32, _i64, _vp, _vp, _vp]
     _lib = lib
@@ -389,6 +397,37 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def file_chunk(count: int, n: int, rank: int = 0, nodes: int = 1) -> tuple[int, int]:
+    """(first scalar, number of scalars) rank `rank` of `nodes` reads from the
+    reference's matrix file (LU-fqhe/src/PermHandler.cpp:97-100)."""
+    first, length = _i64(), _i64()
+    st = load().lufqhe_file_chunk(count, n, rank, nodes, ctypes.byref(first), ctypes.byref(length))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_file_chunk", st)
+    return first.value, length.value
+
+
+def read_matrices(path, count: int, n: int, fmt: str = "complex64", rank: int = 0,
+                  nodes: int = 1, device="cuda", stream: torch.cuda.Stream | None = None):
+    """Read this rank's run of the reference's raw matrix file onto the
+    device.  Returns a complex128 tensor: [m, n, n] when the run holds whole
+    matrices starting at a matrix boundary, else the flat run of complex
+    values (a run of odd length keeps its last real part in an extra slot)."""
+    first, length = file_chunk(count, n, rank, nodes)
+    flat = torch.empty(max(1, (length + 1) // 2), dtype=torch.complex128, device=device)
+    got = _i64()
+    st = _lib.lufqhe_read_matrices(str(path).encode(), FILE_FORMATS[fmt], count, n, rank, nodes,
+                                   _ptr(flat), 2 * flat.numel(), ctypes.byref(got),
+                                   _stream_handle(stream))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_read_matrices", st)
+    vals = got.value
+    el = n * n * 2
+    if first % el == 0 and vals % el == 0:
+        return flat[: vals // 2].view(vals // el, n, n)
+    return flat[: (vals + 1) // 2]
 
 
 MIXED_MODES = {"graph": 0, "streams": 1, "plain": 2}

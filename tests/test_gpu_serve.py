@@ -66,12 +66,12 @@ def test_alone_equals_in_the_mix(lc, dev):
     srv.close()
 
 
-def scaled_sum(mant, exp):
-    """sum mant_t * 2^exp_t as (complex, magnitude sum), in exact rationals."""
+def scaled_sum(mant, exp, e0):
+    """sum mant_t * 2^(exp_t - e0) exactly (Fractions), and sum |det_t| * 2^-e0."""
     from fractions import Fraction
-    re = sum(Fraction(float(m.real)) * Fraction(2) ** int(e) for m, e in zip(mant, exp))
-    im = sum(Fraction(float(m.imag)) * Fraction(2) ** int(e) for m, e in zip(mant, exp))
-    mag = sum(abs(complex(m)) * 2.0 ** int(e) for m, e in zip(mant, exp))
+    re = sum(Fraction(float(m.real)) * Fraction(2) ** (int(e) - e0) for m, e in zip(mant, exp))
+    im = sum(Fraction(float(m.imag)) * Fraction(2) ** (int(e) - e0) for m, e in zip(mant, exp))
+    mag = sum(abs(complex(m)) * 2.0 ** (int(e) - e0) for m, e in zip(mant, exp))
     return re, im, mag
 
 
@@ -90,9 +90,11 @@ def test_sums_match_recomputed_terms(lc, dev):
         idx = (int(r["perm_start"]) + np.arange(int(r["k"]))) % srv.perms_per_walker
         perm = torch.from_numpy(np.ascontiguousarray(tables[w][idx])).to(dev)
         res = lc.fqhe_det(torch.from_numpy(zp[None]).to(dev), row_perm=perm, count=int(r["k"]))
-        re, im, mag = scaled_sum(res.mant.cpu().numpy(), res.exp.cpu().numpy())
-        got_re = Fraction(float(r["sum_mant"].real)) * Fraction(2) ** int(r["sum_exp"])
-        got_im = Fraction(float(r["sum_mant"].imag)) * Fraction(2) ** int(r["sum_exp"])
+        ex = res.exp.cpu().numpy()
+        e0 = int(ex.max())   # everything scaled by 2^-e0 (the determinants overflow doubles)
+        re, im, mag = scaled_sum(res.mant.cpu().numpy(), ex, e0)
+        got_re = Fraction(float(r["sum_mant"].real)) * Fraction(2) ** (int(r["sum_exp"]) - e0)
+        got_im = Fraction(float(r["sum_mant"].imag)) * Fraction(2) ** (int(r["sum_exp"]) - e0)
         err = math.hypot(float(got_re - re), float(got_im - im))
         assert err <= 1e-12 * mag, (int(r["walker"]), err, mag)
         if r["accepted"]:
