@@ -76,15 +76,16 @@ int32_t lufqhe_det_batched(const void *d_A, int64_t batch, int32_t n,
                            int32_t *d_info, int8_t *d_piv, int32_t policy,
                            int32_t arith, hipStream_t stream);
 
-/* Launch shapes.  Every n has up to 3 compiled kernel shapes (lanes per
- * matrix, lane-columns); the autotuner times them on the device and selects
- * one per (policy, arithmetic, n).  Replaces the reference's fixed sizing,
- * RunInfo::RunInfo (LU-fqhe/src/RunInfo.cpp:46-78) and the matrices-per-block
- * command-line option (LU-fqhe/src/FQHE.cpp:214-265).  Selection is process
- * wide; change it while no launch of that n is being issued. */
+/* Launch shapes.  Every n has up to 4 compiled kernel shapes (lanes per
+ * matrix, lane-columns, waves per SIMD the register allocation allows);
+ * the autotuner times them on the device and selects one per (policy,
+ * arithmetic, n).  Replaces the reference's fixed sizing, RunInfo::RunInfo
+ * (LU-fqhe/src/RunInfo.cpp:46-78) and the matrices-per-block command-line
+ * option (LU-fqhe/src/FQHE.cpp:214-265).  Selection is process wide; change
+ * it while no launch of that n is being issued.  min_waves may be NULL. */
 int32_t lufqhe_shape_count(int32_t n);
 int32_t lufqhe_shape_info(int32_t n, int32_t shape, int32_t *lanes,
-                          int32_t *lane_cols);
+                          int32_t *lane_cols, int32_t *min_waves);
 int32_t lufqhe_set_shape(int32_t policy, int32_t arith, int32_t n,
                          int32_t shape);
 /* The selected shape, or -1 for invalid arguments. */

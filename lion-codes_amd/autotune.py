@@ -66,12 +66,12 @@ def tune(policies=("crout_ref", "abs"), ns=range(1, 65), target_mib: int = 256, 
                 times.append(t)
             best = min(range(len(times)), key=times.__getitem__)
             lc.set_shape(policy, "fused", n, best)
-            lanes, cols = lc.shape_info(n, best)
+            lanes, cols, waves = lc.shape_info(n, best)
             entries.append({"policy": policy, "n": n, "shape": best, "lanes": lanes,
-                            "lane_cols": cols, "count": count,
+                            "lane_cols": cols, "waves": waves, "count": count,
                             "us_per_launch": [round(t * 1e3, 2) for t in times],
                             "ns_per_det": round(times[best] * 1e6 / count, 3)})
-            log(f"{policy:9s} n={n:2d} shape={best} (L={lanes}, CG={cols}) "
+            log(f"{policy:9s} n={n:2d} shape={best} (L={lanes}, CG={cols}, waves={waves}) "
                 f"times(us)={[round(t * 1e3, 1) for t in times]}")
             del A, res
     props = torch.cuda.get_device_properties(0)

@@ -173,13 +173,15 @@ int32_t lufqhe_shape_count(int32_t n) {
   return c;
 }
 
-int32_t lufqhe_shape_info(int32_t n, int32_t shape, int32_t* lanes, int32_t* lane_cols) {
+int32_t lufqhe_shape_info(int32_t n, int32_t shape, int32_t* lanes, int32_t* lane_cols,
+                          int32_t* min_waves) {
   if (n < 1 || n > kMaxN || shape < 0 || shape >= kMaxShapes || !lanes || !lane_cols)
     return LUFQHE_ERROR_INVALID_VALUE;
   const KernelEntry& e = g_kernels[0][n][shape];
   if (!e.launch) return LUFQHE_ERROR_NOT_BUILT;
   *lanes = e.shape.lanes;
   *lane_cols = e.shape.lane_cols;
+  if (min_waves) *min_waves = e.shape.min_waves;
   return LUFQHE_SUCCESS;
 }
 

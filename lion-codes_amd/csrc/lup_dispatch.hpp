@@ -32,6 +32,7 @@ struct KernelShape {
   int lanes;
   int lane_cols;
   int waves_per_block;
+  int min_waves;  // waves per SIMD the register allocation allows at least
 };
 
 // Launches the (variant, n) kernel on `blocks` workgroups.

@@ -22,7 +22,7 @@ constexpr int kWavesPerBlock = 4;
 template <int N, int SH>
 hipError_t launch(const BatchArgs& args, int64_t blocks, hipStream_t stream) {
   constexpr ShapeLC sh = candidate_shape(N, SH);
-  hipLaunchKernelGGL((lup_det_kernel<N, sh.lanes, sh.lane_cols, kPolicy, kArith, kWavesPerBlock, 1>),
+  hipLaunchKernelGGL((lup_det_kernel<N, sh.lanes, sh.lane_cols, kPolicy, kArith, kWavesPerBlock, sh.waves>),
                      dim3((unsigned)blocks), dim3(64 * kWavesPerBlock), 0, stream, args);
   return hipGetLastError();
 }
@@ -34,9 +34,9 @@ void register_shape() {
   if constexpr (sh.lanes > 0 && (kArith == ARITH_FUSED || SH == 0)) {
     KernelEntry& e = g_kernels[LUFQHE_VARIANT][N][SH];
     e.launch = &launch<N, SH>;
-    e.shape = KernelShape{sh.lanes, sh.lane_cols, kWavesPerBlock};
+    e.shape = KernelShape{sh.lanes, sh.lane_cols, kWavesPerBlock, sh.waves};
     e.kernel = reinterpret_cast<const void*>(
-        &lup_det_kernel<N, sh.lanes, sh.lane_cols, kPolicy, kArith, kWavesPerBlock, 1>);
+        &lup_det_kernel<N, sh.lanes, sh.lane_cols, kPolicy, kArith, kWavesPerBlock, sh.waves>);
   }
 }
 
@@ -46,6 +46,7 @@ void register_one() {
     register_shape<N, 0>();
     register_shape<N, 1>();
     register_shape<N, 2>();
+    register_shape<N, 3>();
   }
 }
 

@@ -713,26 +713,51 @@ __global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) 
   }
 }
 
-// Candidate launch shapes per n, as (lanes per matrix L, lane-columns CG).
-// Every candidate is compiled; the autotuner picks one per n on the device
-// and candidate 0 is the default before tuning.  The lists come from shape
-// sweeps on MI355X: small n wants many matrices per wave (even two rows per
-// lane), mid n a split of each row over lane-columns, n > 32 one matrix per
-// wave.
+// Candidate launch shapes per n, as (lanes per matrix L, lane-columns CG,
+// waves per SIMD the register allocation must allow).  Every candidate is
+// compiled; the autotuner picks one per n on the device and candidate 0 is
+// the default before tuning.  The lists come from shape sweeps on MI355X:
+// small n wants many matrices per wave (even two rows per lane), mid n a
+// split of each row over lane-columns, n > 40 one matrix per wave.  From
+// n = 41 the tile alone needs 100-256 VGPRs and the natural allocation
+// leaves one wave per SIMD, which cannot hide the step's dependent
+// latencies; the waves = 2 candidates cap the allocation at 256 registers
+// (spilling the coldest values to scratch), which measured up to 1.6x
+// faster at n = 48.
 struct ShapeLC {
   int lanes;
   int lane_cols;
+  int waves;
 };
-constexpr int kMaxShapes = 3;
+constexpr int kMaxShapes = 4;
 constexpr ShapeLC candidate_shape(int n, int idx) {
-  if (n <= 4) return idx == 0 ? ShapeLC{4, 1} : idx == 1 ? ShapeLC{8, 1} : ShapeLC{0, 0};
-  if (n <= 8) return idx == 0 ? ShapeLC{4, 1} : idx == 1 ? ShapeLC{8, 1} : ShapeLC{0, 0};
-  if (n <= 16) return idx == 0 ? ShapeLC{16, 1} : idx == 1 ? ShapeLC{8, 1} : ShapeLC{0, 0};
-  if (n <= 24) return idx == 0 ? ShapeLC{32, 4} : idx == 1 ? ShapeLC{32, 1} : ShapeLC{32, 2};
-  if (n <= 32) return idx == 0 ? ShapeLC{32, 2} : idx == 1 ? ShapeLC{32, 4} : ShapeLC{32, 1};
-  if (n <= 40) return idx == 0 ? ShapeLC{64, 4} : idx == 1 ? ShapeLC{64, 1} : ShapeLC{0, 0};
-  if (n <= 48) return idx == 0 ? ShapeLC{64, 4} : idx == 1 ? ShapeLC{64, 2} : ShapeLC{0, 0};
-  return idx == 0 ? ShapeLC{64, 2} : idx == 1 ? ShapeLC{64, 4} : ShapeLC{0, 0};
+  constexpr ShapeLC none{0, 0, 0};
+  if (n <= 8) {
+    constexpr ShapeLC c[] = {{4, 1, 1}, {8, 1, 1}, none, none};
+    return c[idx];
+  }
+  if (n <= 16) {
+    constexpr ShapeLC c[] = {{16, 1, 1}, {8, 1, 1}, none, none};
+    return c[idx];
+  }
+  if (n <= 24) {
+    constexpr ShapeLC c[] = {{32, 4, 1}, {16, 2, 1}, {32, 2, 1}, none};
+    return c[idx];
+  }
+  if (n <= 32) {
+    constexpr ShapeLC c[] = {{32, 2, 1}, {32, 4, 1}, {32, 1, 1}, none};
+    return c[idx];
+  }
+  if (n <= 40) {
+    constexpr ShapeLC c[] = {{64, 4, 1}, {64, 1, 1}, {64, 4, 2}, none};
+    return c[idx];
+  }
+  if (n <= 48) {
+    constexpr ShapeLC c[] = {{64, 4, 2}, {64, 2, 2}, {64, 4, 1}, {32, 4, 1}};
+    return c[idx];
+  }
+  constexpr ShapeLC c[] = {{64, 2, 1}, {64, 4, 1}, {64, 4, 2}, {64, 2, 2}};
+  return c[idx];
 }
 
 }  // namespace lufqhe

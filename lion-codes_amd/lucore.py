@@ -79,7 +79,8 @@ def load() -> ctypes.CDLL:
     lib.lufqhe_shape_count.restype = _i32
     lib.lufqhe_shape_count.argtypes = [_i32]
     lib.lufqhe_shape_info.restype = _i32
-    lib.lufqhe_shape_info.argtypes = [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]
+    lib.lufqhe_shape_info.argtypes = [_i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                      ctypes.POINTER(_i32)]
     lib.lufqhe_set_shape.restype = _i32
     lib.lufqhe_set_shape.argtypes = [_i32, _i32, _i32, _i32]
     lib.lufqhe_get_shape.restype = _i32
@@ -131,7 +132,13 @@ def apply_shape_table(path: Path | None = None) -> int:
     table = json.loads(p.read_text())
     applied = 0
     for e in table.get("entries", []):
-        if set_shape(e["policy"], "fused", int(e["n"]), int(e["shape"])):
+        n, sh = int(e["n"]), int(e["shape"])
+        # an entry applies only to the candidate it was measured on
+        if not 0 <= sh < shape_count(n):
+            continue
+        if shape_info(n, sh) != (e["lanes"], e["lane_cols"], e.get("waves", 1)):
+            continue
+        if set_shape(e["policy"], "fused", n, sh):
             applied += 1
     return applied
 
@@ -140,13 +147,15 @@ def shape_count(n: int) -> int:
     return load().lufqhe_shape_count(n)
 
 
-def shape_info(n: int, shape: int) -> tuple[int, int]:
-    """(lanes per matrix, lane-columns) of candidate `shape` for size n."""
-    lanes, cols = _i32(), _i32()
-    st = load().lufqhe_shape_info(n, shape, ctypes.byref(lanes), ctypes.byref(cols))
+def shape_info(n: int, shape: int) -> tuple[int, int, int]:
+    """(lanes per matrix, lane-columns, waves per SIMD the register
+    allocation allows) of candidate `shape` for size n."""
+    lanes, cols, waves = _i32(), _i32(), _i32()
+    st = load().lufqhe_shape_info(n, shape, ctypes.byref(lanes), ctypes.byref(cols),
+                                ctypes.byref(waves))
     if st != SUCCESS:
         raise LufqheError("lufqhe_shape_info", st)
-    return lanes.value, cols.value
+    return lanes.value, cols.value, waves.value
 
 
 def set_shape(policy: str, arith: str, n: int, shape: int) -> bool:

@@ -75,10 +75,11 @@ def test_shape_table_api(lc):
     lib = lc.load()
     for n in range(1, 65):
         c = lib.lufqhe_shape_count(n)
-        assert 1 <= c <= 3
+        assert 1 <= c <= 4
         for sh in range(c):
-            lanes, cols = lc.shape_info(n, sh)
+            lanes, cols, waves = lc.shape_info(n, sh)
             assert lanes in (4, 8, 16, 32, 64) and cols in (1, 2, 4) and lanes % cols == 0
+            assert waves in (1, 2)
     assert lib.lufqhe_shape_count(0) == 0 and lib.lufqhe_shape_count(65) == 0
     assert lib.lufqhe_set_shape(0, 0, 32, 7) == 1          # shape out of range
     assert lib.lufqhe_set_shape(0, 1, 32, 1) == 4          # reference arithmetic: shape 0 only
