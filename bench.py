@@ -34,7 +34,7 @@ OUT_BYTES = 16 + 4 + 8 + 8 + 4   # mant, exp, logabs, phase, info
 # HBM bytes one launch of the n=32 kernel moves, measured with rocprofv3
 # (FETCH_SIZE and WRITE_SIZE, one counter per run, each scaled by what the
 # same counter reads for a plain 1 GiB device copy: x2.0 and x1.0).
-MEASURED_TRAFFIC_N32 = 1.0772e9
+MEASURED_TRAFFIC_N32 = 1.0773e9
 
 
 def det_bytes(n: int) -> int:
@@ -252,6 +252,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the mixed-size, FQHE and serving measurements")
+    ap.add_argument("--legs", default="mixed,fqhe,serving",
+                    help="comma list of extra measurements to run (mixed, fqhe, serving)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -328,14 +330,18 @@ def main() -> None:
     if not args.no_cpu:
         A_host = A[:16384].cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(A_host, host_cores())
-    if not args.no_extra:
+    legs = [] if args.no_extra else [x for x in args.legs.split(",") if x]
+    if legs:
         del A
         torch.cuda.empty_cache()
         reps = max(3, min(args.steps, 10))
+    if "mixed" in legs:
         print("# mixed-size batch (config 4)", file=sys.stderr, flush=True)
         out["mixed"] = bench_mixed(lc, wl, dev, reps)
+    if "fqhe" in legs:
         print("# FQHE batch (config 3)", file=sys.stderr, flush=True)
         out["fqhe"] = bench_fqhe(lc, wl, dev, reps)
+    if "serving" in legs:
         print("# serving (config 5)", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(lc, dev)
     print(json.dumps(out), flush=True)
