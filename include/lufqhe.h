@@ -90,6 +90,12 @@ int32_t lufqhe_set_shape(int32_t policy, int32_t arith, int32_t n,
                          int32_t shape);
 /* The selected shape, or -1 for invalid arguments. */
 int32_t lufqhe_get_shape(int32_t policy, int32_t arith, int32_t n);
+/* Applies a shape table written by the autotuner (JSON; path NULL:
+ * $LUFQHE_SHAPE_TABLE, else shapes_gfx950.json next to the library).  An
+ * entry is applied only if the candidate at its index still has the
+ * recorded (lanes, lane_cols, waves).  Returns the number of entries
+ * applied, or -1 if the file cannot be read or holds no entry list. */
+int32_t lufqhe_load_shape_table(const char *path);
 
 /* Bytes of device workspace lufqhe_det_inplace needs for `count` matrices
  * (one mantissa and one exponent per matrix). */

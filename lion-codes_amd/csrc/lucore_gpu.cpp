@@ -1,6 +1,7 @@
 // lucore_gpu.cpp -- reference-shaped C++ host API over the C ABI.
 #include "../../include/lucore_gpu.hpp"
 
+#include <mutex>
 #include <string>
 
 RunInfo::RunInfo(int nMperBl, int matSide, int nElem, bool padded, int nodes)
@@ -41,6 +42,10 @@ void setPivotPolicy(int policy) { t_policy = policy; }
 void CALL(double2** d_Matrices, int nMat, RunInfo* rInfo, int rank, int th_num,
           hipStream_t* stream) {
   if (!d_Matrices || !rInfo || !stream) check(LUFQHE_ERROR_INVALID_VALUE, "CALL", rank, th_num);
+  // the tuned launch shapes, once per process ($LUFQHE_SHAPE_TABLE or the
+  // table next to the library; without one the default shapes run)
+  static std::once_flag table_once;
+  std::call_once(table_once, [] { (void)lufqhe_load_shape_table(nullptr); });
   const int n = rInfo->getMatSide();
   const int ld = rInfo->getLoadSize();
   t_stride = (long long)n * ld;

@@ -115,6 +115,8 @@ def load() -> ctypes.CDLL:
     lib.lufqhe_read_matrices.restype = _i32
     lib.lufqhe_read_matrices.argtypes = [ctypes.c_char_p, _i32, _i64, _i32, _i32, _i32, _vp, _i64,
                                          ctypes.POINTER(_i64), _vp]
+    lib.lufqhe_load_shape_table.restype = _i32
+    lib.lufqhe_load_shape_table.argtypes = [ctypes.c_char_p]
     lib.lufqhe_fqhe_build.restype = _i32
     lib.lufqhe_fqhe_build.argtypes = [_vp, _i64, _i32, _i64, _vp, _vp, _vp]
     _lib = lib

@@ -101,3 +101,29 @@ def test_fqhe_argument_validation(lc):
                                1, 1, None) == 2                      # ABS + reference arithmetic
     assert lib.lufqhe_fqhe_build(None, 4, 8, 8, None, None, None) == 1
     assert lib.lufqhe_fqhe_build(None, 0, 8, 8, None, None, None) == 0
+
+
+def test_shape_table_loads_from_c(lc, tmp_path):
+    """The C loader applies the committed table exactly as the Python one."""
+    import json
+    lib = lc.load()
+    from importlib import import_module
+    core = import_module("lion-codes_amd.lucore")
+    entries = json.loads(core.SHAPE_TABLE.read_text())["entries"]
+    saved = {(e["policy"], e["n"]): lc.get_shape(e["policy"], "fused", e["n"]) for e in entries}
+    try:
+        for (pol, n) in saved:
+            assert lc.set_shape(pol, "fused", n, 0)
+        applied = lib.lufqhe_load_shape_table(str(core.SHAPE_TABLE).encode())
+        assert applied == len(entries)
+        for e in entries:
+            assert lc.get_shape(e["policy"], "fused", e["n"]) == e["shape"]
+        # an entry whose geometry no longer matches its index is skipped
+        bad = dict(entries[0], lanes=entries[0]["lanes"] * 2 if entries[0]["lanes"] < 64 else 4)
+        p = tmp_path / "t.json"
+        p.write_text(json.dumps({"entries": [bad]}))
+        assert lib.lufqhe_load_shape_table(str(p).encode()) == 0
+        assert lib.lufqhe_load_shape_table(str(tmp_path / "missing.json").encode()) == -1
+    finally:
+        for (pol, n), sh in saved.items():
+            lc.set_shape(pol, "fused", n, sh)
