@@ -118,6 +118,23 @@ def log_err_vs(la, ph, la_ref, ph_ref):
     return np.abs(np.expm1((la - la_ref) + 1j * np.angle(np.exp(1j * (ph - ph_ref)))))
 
 
+def bench_c1(lc, wl, dev, reps: int) -> dict:
+    """Config 1: 4096 complex128 16x16 matrices -- a batch small enough that
+    one launch is latency-bound; reported as launch time and det/s."""
+    import torch
+    from oracle import oracle
+    A = wl.gaussian(4096, 16, 11, device=dev)
+    res = lc.alloc_result(4096, 16, dev)
+    s = torch.cuda.current_stream()
+    lc.det_batched(A, out=res)
+    ms = time_events(lambda: lc.det_batched(A, out=res), s, reps)
+    o = oracle.crout_ref(A.cpu().numpy())
+    rel = np.abs(res.det().cpu().numpy() / o["det"] - 1.0)
+    return {"workload": "4096 complex128 16x16, re,im ~ N(0,1), seed 11, one launch",
+            "ms": ms, "det_per_s": 4096 / (ms * 1e-3),
+            "vs_crout_ref_max_rel": float(rel.max())}
+
+
 def bench_mixed(lc, wl, dev, reps: int) -> dict:
     """Config 4: 200000 matrices, n ~ U{8..64}, 5% near-singular; one
     hipGraph replay of the bucketed plan vs per-size plain launches."""
@@ -327,6 +344,7 @@ def main() -> None:
                      "bytes_per_det": det_bytes(N), "flops_per_det": det_flops(N)},
     }
     out["accuracy"] = accuracy(lc, A, res)
+    out["c1"] = bench_c1(lc, wl, dev, max(5, args.steps))
     if not args.no_cpu:
         A_host = A[:16384].cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(A_host, host_cores())

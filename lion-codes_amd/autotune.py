@@ -1,11 +1,13 @@
 """Autotuner of the kernel launch shapes.
 
-Every size n has up to three compiled kernel shapes (lanes per matrix,
-lane-columns; ``candidate_shape`` in csrc/lup_kernel.hpp).  This times each
-one on the device on a seeded N(0,1) batch of about ``target_mib`` MiB,
-checks that it returns the same determinants bit for bit as shape 0, and
-writes the fastest per (policy, n) to a JSON table that the host binding
-applies when it loads the library (lucore.SHAPE_TABLE).
+Every size n has up to four compiled kernel shapes (lanes per matrix,
+lane-columns, waves per SIMD the register allocation allows;
+``candidate_shape`` in csrc/lup_kernel.hpp).  This times each one on the
+device on a seeded N(0,1) batch of about ``target_mib`` MiB, checks that it
+returns the same determinants bit for bit as shape 0, and writes the fastest
+per (policy, n) to a JSON table.  The Python binding applies the table when
+it loads the library (lucore.SHAPE_TABLE); C and C++ callers apply it with
+lufqhe_load_shape_table, which LUCoreGPU::CALL does on first use.
 
     python lion-codes_amd/autotune.py [--out PATH] [--ns 8,16,32] [--reps 5]
 """

@@ -3,28 +3,32 @@
 // Replaces LUCoreGPU::LUDecomposition (LU-fqhe/src/LUCoreGPU.cu:159-437) and
 // its reductions vOP::* (LU-fqhe/src/vectorOP.cu:23-372).
 //
-// Layout: one matrix row per lane, the whole row resident in VGPRs.  A wave64
-// is split into G = 64/L groups of L lanes (L = 8, 16, 32 or 64, L >= n); group
-// g of the wave factors one matrix, lane r of the group owns row r.  Rows are
-// never moved: a lane keeps its row for the whole factorisation and carries
-// the row's *reference position* `pos`, which is what the reference's physical
-// row swaps (LU-fqhe/src/LUCoreCPU.cu:215-232) would have made of it.
+// Layout (see Tile): a group of L lanes factors one matrix, a wave64 holds
+// G = 64/L groups.  The group's lanes form L/CG lane-rows by CG lane-columns;
+// rows and columns are dealt to them cyclically and the whole matrix stays in
+// VGPRs.  Rows never move: each row slot carries the row's *reference
+// position*, which is what the reference's physical row swaps
+// (LU-fqhe/src/LUCoreCPU.cu:215-232) would have made of it.
 //
-// Step k of the right-looking factorisation:
+// Step k of the right-looking factorisation (factor_tile):
 //   1. criterion c on column k (per policy), -1 on finished rows;
-//   2. exact pivot choice: cmax = group max of c, then the largest `pos`
-//      among lanes with c == cmax (the reference's `>=` scan keeps the last
-//      position among exact ties, LU-fqhe/src/LUCoreCPU.cu:207-212);
+//   2. exact pivot choice: cmax = group max of c, then the largest position
+//      among candidates with c == cmax (the reference's `>=` scan keeps the
+//      last position among exact ties, LU-fqhe/src/LUCoreCPU.cu:207-212);
 //      butterflies use DPP (quad_perm, row_half_mirror, row_mirror) and the
 //      gfx950 v_permlane16_swap / v_permlane32_swap across 16-lane rows;
-//   3. the pivot lane writes its row (columns k..n-1) to the group's LDS slot,
-//      every lane reads it back (LDS broadcast);
+//      this search is woven into step k-1's update (update_and_search);
+//   3. the pivot's lane-row writes the pivot row (columns k..n-1) to the
+//      group's LDS slot and every lane reads its columns back;
 //   4. det *= pivot (complex, renormalised to mant * 2^exp);
-//   5. remaining rows form their multiplier and apply the rank-1 update to
-//      columns k+1..n-1 (complex FP64, 4 FMAs per element in ARITH_FUSED);
-//   6. the position bookkeeping of the reference swap; the pivot lane retires.
-// The k loop is fully unrolled (N is a template constant) so that done
-// columns are skipped statically and the row array stays in registers.
+//   5. multipliers l = a_ik / p (lane-row DPP broadcast of column k) and the
+//      rank-1 update of columns k+1..n-1 (complex FP64, 4 FMAs per element in
+//      ARITH_FUSED);
+//   6. the position bookkeeping of the reference swap; the pivot row retires.
+// The k loop is fully unrolled (N is a template constant) so that finished
+// column slots are skipped statically and the tile stays in registers.  The
+// load phase reads the matrix from HBM, or builds an FQHE matrix from
+// particle positions (SRC_FQHE).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -164,17 +168,6 @@ struct Tile {
   static constexpr int NPAD = C * CG;
   static constexpr int G = 64 / L;
 };
-
-// Compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1, so the
-// body sees its index as a constant (static register indices, DPP controls).
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
 
 constexpr int quad_perm(int a, int b, int c, int d) {
   return a | (b << 2) | (c << 4) | (d << 6);
