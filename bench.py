@@ -118,6 +118,26 @@ def log_err_vs(la, ph, la_ref, ph_ref):
     return np.abs(np.expm1((la - la_ref) + 1j * np.angle(np.exp(1j * (ph - ph_ref)))))
 
 
+def bench_h2d(lc, A, res, reps: int) -> dict:
+    """The same n=32 batch handed over in pinned host memory, as the
+    reference's loop does (a host-to-device copy before every CALL,
+    LU-fqhe/src/MonteCarloLoop.cu:316): copy + launch, HIP events."""
+    import torch
+    host = torch.empty(A.shape, dtype=A.dtype, pin_memory=True)
+    host.copy_(A)
+    dev_buf = torch.empty_like(A)
+    s = torch.cuda.current_stream()
+
+    def step():
+        dev_buf.copy_(host, non_blocking=True)
+        lc.det_batched(dev_buf, out=res)
+    step()
+    ms = time_events(step, s, reps)
+    return {"ms": ms, "det_per_s": A.shape[0] / (ms * 1e-3),
+            "h2d_gb_per_s": A.numel() * 16 / (ms * 1e-3) / 1e9,
+            "note": "PCIe-inclusive rate of the n=32 batch; not the headline value"}
+
+
 def bench_c1(lc, wl, dev, reps: int) -> dict:
     """Config 1: 4096 complex128 16x16 matrices -- a batch small enough that
     one launch is latency-bound; reported as launch time and det/s."""
@@ -345,6 +365,7 @@ def main() -> None:
     }
     out["accuracy"] = accuracy(lc, A, res)
     out["c1"] = bench_c1(lc, wl, dev, max(5, args.steps))
+    out["h2d_inclusive"] = bench_h2d(lc, A, res, 5)
     if not args.no_cpu:
         A_host = A[:16384].cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(A_host, host_cores())

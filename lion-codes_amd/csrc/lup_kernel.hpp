@@ -139,6 +139,13 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Select by value.  The arguments are copies, so no pointer into the tile is
+// ever selected (a select of two addresses in the tile would keep the whole
+// tile out of registers).
+__device__ __forceinline__ double2 sel2(bool c, double2 x, double2 y) {
+  return make_double2(c ? x.x : y.x, c ? x.y : y.y);
+}
+
 // mant * 2^e renormalisation, exact (power-of-two scaling).
 __device__ __forceinline__ void renorm(double& x, double& y, int& e) {
   const double m = fmax(fabs(x), fabs(y));
@@ -467,15 +474,35 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
 #pragma unroll
     for (int t = 0; t < R; ++t) is_piv[t] = (pos[t] == ppos);
 
-    // The pivot's lane-row writes the pivot row (columns >= k) to LDS: one
-    // exec-masked region per row slot, so no data selects are needed and a
-    // region whose mask is empty is branched over.  The wave executes its
-    // LDS instructions in order, so the reads below see these writes.
+    // The pivot's lane-row writes the pivot row (columns >= k) to LDS.  A
+    // ds_write costs the same however few lanes are active, so:
+    //  * one matrix per wave: one exec-masked region per row slot; only the
+    //    pivot's region runs (empty regions are branched over);
+    //  * several matrices per wave: their pivots sit in different row slots,
+    //    so the pivot lanes first select their row's slot values and one
+    //    region writes every group's pivot row at once.
+    // The wave executes its LDS instructions in order, so the reads below see
+    // these writes.
+    if constexpr (T::G > 1 && R > 1) {
+      bool any = false;
 #pragma unroll
-    for (int t = 0; t < R; ++t) {
-      if (is_piv[t]) {
+      for (int t = 0; t < R; ++t) any = any || is_piv[t];
+      if (any) {
 #pragma unroll
-        for (int s = sk; s < C; ++s) bc[cb + s * CG] = a[t][s];
+        for (int s = sk; s < C; ++s) {
+          double2 v = a[0][s];
+#pragma unroll
+          for (int t = 1; t < R; ++t) v = sel2(is_piv[t], a[t][s], v);
+          bc[cb + s * CG] = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        if (is_piv[t]) {
+#pragma unroll
+          for (int s = sk; s < C; ++s) bc[cb + s * CG] = a[t][s];
+        }
       }
     }
     wave_lds_sync();
