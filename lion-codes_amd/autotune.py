@@ -49,6 +49,16 @@ def tune(policies=("crout_ref", "abs"), ns=range(1, 65), target_mib: int = 256, 
     wl = importlib.import_module("lion-codes_amd.workloads")
     lc.load()
     dev = torch.device("cuda:0")
+    # keep the device busy for a while first: its clocks ramp up over the
+    # first tens of milliseconds of work, which would favour later shapes
+    A = wl.gaussian(65536, 32, seed, device=dev)
+    res = lc.alloc_result(65536, 32, dev)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.3:
+        for _ in range(10):
+            lc.det_batched(A, out=res)
+        torch.cuda.synchronize()
+    del A, res
     entries = []
     for policy in policies:
         for n in ns:
