@@ -17,7 +17,9 @@ namespace {
 
 constexpr int kPolicy = variant_policy(LUFQHE_VARIANT);
 constexpr int kArith = variant_arith(LUFQHE_VARIANT);
-constexpr int kWavesPerBlock = 4;
+// One wave per workgroup: measured 3-11% faster than 2 or 4 on MI355X
+// (finer-grained placement of the waves as others finish).
+constexpr int kWavesPerBlock = 1;
 
 template <int N, int SH>
 hipError_t launch(const BatchArgs& args, int64_t blocks, hipStream_t stream) {
