@@ -263,6 +263,10 @@ def bench_serving(lc, dev) -> dict:
                   hbm_budget=200 * 10 ** 9, records=False)
     srv.run(requests=1024, rate=0.0, seed=1, **common)   # warm-up
     cap, _ = srv.run(requests=8192, rate=0.0, seed=2, **common)
+    # the same kind of request stream served one request per batch, one
+    # batch at a time: per-request launches without continuous batching
+    one = dict(common, max_batch_requests=1, max_inflight=1)
+    plain, _ = srv.run(requests=1536, rate=0.0, seed=3, **one)
     out = {"workload": "512 walkers (n = 8,16,24,32,48,64 in turn), FQHE entries, one particle "
                        "moved per step (complex Gaussian, sigma 0.3), k ~ U[1,2000] "
                        "row-permutation terms per request, 2000 permutations per walker, "
@@ -270,6 +274,9 @@ def bench_serving(lc, dev) -> dict:
            "capacity": {"det_per_s": cap["dets_per_s"], "requests_per_s": cap["requests_per_s"],
                         "p50_ms": cap["p50_s"] * 1e3, "p99_ms": cap["p99_s"] * 1e3,
                         "mean_batch_dets": cap["mean_batch_dets"]},
+           "one_request_per_launch": {"det_per_s": plain["dets_per_s"],
+                                      "requests_per_s": plain["requests_per_s"]},
+           "speedup_batched_vs_one_request_per_launch": cap["dets_per_s"] / plain["dets_per_s"],
            "sweep": []}
     for frac in (0.25, 0.5, 0.8, 0.95):
         rate = frac * cap["requests_per_s"]
