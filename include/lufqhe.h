@@ -179,9 +179,11 @@ int32_t lufqhe_fqhe_build(const void *d_z, int64_t count, int32_t n,
  * builds the FQHE matrix on the device (as lufqhe_fqhe_det).  Modes:
  *   LUFQHE_MIXED_GRAPH    the bucket launches forked over the plan streams,
  *                         captured into one hipGraph on the first call for a
- *                         given set of pointers (instantiation allocates) and
- *                         replayed after; `stream` must not be the legacy
- *                         default stream (capture is not allowed there);
+ *                         given set of pointers and source (instantiation
+ *                         allocates) and replayed after -- with the launch
+ *                         shapes selected at capture time; `stream` must not
+ *                         be the legacy default stream (capture is not
+ *                         allowed there);
  *   LUFQHE_MIXED_STREAMS  the same forked launches, issued directly;
  *   LUFQHE_MIXED_PLAIN    one launch per size, one after another on `stream`.
  * A plan is not re-entrant: one execute at a time per plan. */
