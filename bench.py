@@ -141,8 +141,8 @@ def bench_h2d(lc, A, res, reps: int) -> dict:
             "note": "PCIe-inclusive rate of the n=32 batch; not the headline value"}
 
 
-def bench_c1(lc, wl, dev, reps: int) -> dict:
-    """Config 1: 4096 complex128 16x16 matrices -- a batch small enough that
+def bench_small_batch(lc, wl, dev, reps: int) -> dict:
+    """4096 complex128 16x16 matrices -- a batch small enough that
     one launch is latency-bound; reported as launch time and det/s."""
     import torch
     from oracle import oracle
@@ -159,7 +159,7 @@ def bench_c1(lc, wl, dev, reps: int) -> dict:
 
 
 def bench_mixed(lc, wl, dev, reps: int) -> dict:
-    """Config 4: 200000 matrices, n ~ U{8..64}, 5% near-singular; one
+    """Mixed-size batch: 200000 matrices, n ~ U{8..64}, 5% near-singular; one
     hipGraph replay of the bucketed plan vs per-size plain launches."""
     import torch
     from oracle import oracle
@@ -221,7 +221,7 @@ def bench_mixed(lc, wl, dev, reps: int) -> dict:
 
 
 def bench_fqhe(lc, wl, dev, reps: int) -> dict:
-    """Config 3: 32768 FQHE matrices, n ~ U{10..40}, built on the device from
+    """FQHE batch: 32768 FQHE matrices, n ~ U{10..40}, built on the device from
     particle positions inside the determinant kernels."""
     import torch
     count, seed = 32768, 3
@@ -255,7 +255,7 @@ def bench_fqhe(lc, wl, dev, reps: int) -> dict:
 
 
 def bench_serving(lc, dev) -> dict:
-    """Config 5: 512 walkers, n in {8,16,24,32,48,64}, k ~ U[1,2000] terms per
+    """Walker serving: 512 walkers, n in {8,16,24,32,48,64}, k ~ U[1,2000] terms per
     request, Poisson arrivals swept up to the measured capacity."""
     sizes = [8, 16, 24, 32, 48, 64]
     srv = lc.Server([sizes[w % 6] for w in range(512)], perms_per_walker=2000, seed=7)
@@ -306,8 +306,8 @@ def main() -> None:
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; a batch shards with no "
-                          "communication, see DESIGN.md -- run with --gpus 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU (a batch would shard by "
+                          "matrix ranges with no communication) -- run with --gpus 1"}))
         sys.exit(2)
 
     import torch
@@ -373,7 +373,7 @@ def main() -> None:
         "dtype": "complex128",
         "data": "synthetic",
         "config": {"workload": f"uniform batch: {BATCH} complex128 {N}x{N}, re,im ~ N(0,1), "
-                               f"one plain launch (config 2)", "batch": BATCH, "n": N,
+                               f"one plain launch", "batch": BATCH, "n": N,
                    "pivot_policy": "crout_ref", "arith": "fused", "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_N32,
@@ -383,7 +383,7 @@ def main() -> None:
                      "bytes_per_det": det_bytes(N), "flops_per_det": det_flops(N)},
     }
     out["accuracy"] = accuracy(lc, A, res)
-    out["c1"] = bench_c1(lc, wl, dev, max(5, args.steps))
+    out["small_batch_n16"] = bench_small_batch(lc, wl, dev, max(5, args.steps))
     out["h2d_inclusive"] = bench_h2d(lc, A, res, 5)
     if not args.no_cpu:
         A_host = A[:16384].cpu().numpy()
@@ -394,13 +394,13 @@ def main() -> None:
         torch.cuda.empty_cache()
         reps = max(3, min(args.steps, 10))
     if "mixed" in legs:
-        print("# mixed-size batch (config 4)", file=sys.stderr, flush=True)
+        print("# mixed-size batch", file=sys.stderr, flush=True)
         out["mixed"] = bench_mixed(lc, wl, dev, reps)
     if "fqhe" in legs:
-        print("# FQHE batch (config 3)", file=sys.stderr, flush=True)
+        print("# FQHE batch", file=sys.stderr, flush=True)
         out["fqhe"] = bench_fqhe(lc, wl, dev, reps)
     if "serving" in legs:
-        print("# serving (config 5)", file=sys.stderr, flush=True)
+        print("# walker serving", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(lc, dev)
     print(json.dumps(out), flush=True)
 

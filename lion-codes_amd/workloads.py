@@ -1,17 +1,18 @@
-"""Seeded synthetic workloads of the five benchmark configurations.
+"""Seeded synthetic workloads of the benchmarks and tests.
 
 All generators are deterministic functions of their seed.  Gaussian entries
 use torch's counter-based (Philox) generator, so a batch generated on the GPU
 is reproducible; numpy variants exist for small CPU-side cases.
 
-  C1  4096 x n=16, re,im ~ N(0,1)
-  C2  65536 x n=32, re,im ~ N(0,1)
-  C3  32768 FQHE matrices, n ~ U{10..40}: z_i uniform in the disk of radius
-      sqrt(2n), A_ij = z_i^j exp(-|z_i|^2/4); log|det| is known analytically
-  C4  200000 x n ~ U{8..64}, N(0,1), 5% near-singular (one row replaced by
-      alpha*row_a + beta*row_b + 1e-8*N(0,1))
-  C5  walker configurations of n in {8,16,24,32,48,64} FQHE particles (the
-      serving workload lives in serve.py)
+  small batch     4096 x n=16, re,im ~ N(0,1)
+  uniform batch   65536 x n=32, re,im ~ N(0,1)
+  FQHE batch      32768 FQHE matrices, n ~ U{10..40}: z_i uniform in the
+                  disk of radius sqrt(2n), A_ij = z_i^j exp(-|z_i|^2/4);
+                  log|det| is known analytically
+  mixed batch     200000 x n ~ U{8..64}, N(0,1), 5% near-singular (one row
+                  replaced by alpha*row_a + beta*row_b + 1e-8*N(0,1))
+  walker serving  FQHE walkers of n in {8,16,24,32,48,64} particles; the
+                  walk itself runs on the device (lucore.Server)
 """
 from __future__ import annotations
 

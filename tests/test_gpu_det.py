@@ -137,8 +137,8 @@ def test_nullable_outputs(lc, dev, wl):
 
 @pytest.mark.parametrize("n,count", [(16, 4096), (32, 65536)])
 def test_full_size_uniform_configs(lc, orc, wl, dev, n, count):
-    """The C1/C2 batches at full size: every status clean, a random sample
-    bit-exact against the mirror, all of them within tolerance of the
+    """The 4096 x 16x16 and 65536 x 32x32 batches at full size: every status
+    clean, a random sample bit-exact against the mirror, all of them within tolerance of the
     independent rocSOLVER/MAGMA slogdet, and the accuracy against the
     long-double truth reported on a sample."""
     A = wl.gaussian(count, n, seed=77 + n, device=dev)

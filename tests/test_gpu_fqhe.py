@@ -6,7 +6,7 @@ their load phase.  Checks:
     matrices (lufqhe_fqhe_build), for every launch shape of each n;
   * the materialised matrices equal the sequential-power host build to
     rounding, and the mirror oracle's determinant bit for bit;
-  * C3 at full size (32768 matrices, n ~ U{10..40}) through the mixed plan
+  * the FQHE batch at full size (32768 matrices, n ~ U{10..40}) through the mixed plan
     against the analytic log|det| + i arg det (bounds in that test);
   * row permutations (the symmetrisation terms) change the phase by the
     permutation's sign only.
@@ -59,7 +59,7 @@ def test_fused_build_equals_materialised(lc, orc, wl, dev, n):
 
 @pytest.mark.parametrize("policy", ["crout_ref", "abs"])
 def test_config3_analytic_log_form(lc, orc, wl, dev, policy):
-    """C3: 32768 FQHE matrices, n ~ U{10..40}, seeded; one mixed-plan graph
+    """The FQHE batch: 32768 FQHE matrices, n ~ U{10..40}, seeded; one mixed-plan graph
     launch with the FQHE source.
 
     The monomial matrices of 10..40 random points in a disk are ill

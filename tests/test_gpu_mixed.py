@@ -76,7 +76,7 @@ def test_mixed_graph_rejects_default_stream(lc, wl, dev):
 
 
 def test_config4_full_size(lc, orc, wl, dev):
-    """C4 at full size: 200000 matrices, n ~ U{8..64}, 5% near-singular.
+    """The mixed-size batch at full size: 200000 matrices, n ~ U{8..64}, 5% near-singular.
     Statuses clean, a sample bit-exact against the mirror, the rest against
     the long-double truth by matrix class."""
     flat, ns, offs, sing = wl.mixed_batch(200000, seed=2024, device=dev)
