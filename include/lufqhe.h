@@ -186,7 +186,8 @@ int32_t lufqhe_fqhe_build(const void *d_z, int64_t count, int32_t n,
  *                         allowed there);
  *   LUFQHE_MIXED_STREAMS  the same forked launches, issued directly;
  *   LUFQHE_MIXED_PLAIN    one launch per size, one after another on `stream`.
- * A plan is not re-entrant: one execute at a time per plan. */
+ * A plan is not re-entrant: one execute at a time per plan, on the device
+ * that was current when it was created (else LUFQHE_ERROR_INVALID_VALUE). */
 #define LUFQHE_MIXED_GRAPH 0
 #define LUFQHE_MIXED_STREAMS 1
 #define LUFQHE_MIXED_PLAIN 2
@@ -259,7 +260,8 @@ int32_t lufqhe_read_matrices(const char *path, int32_t format, int64_t count,
  * Results per request go to `records` (may be NULL); the walkers keep their
  * final configurations (lufqhe_server_reset restores the initial ones).
  * The run allocates its batch buffers (hipMalloc, pinned host memory) and
- * frees them before returning. */
+ * frees them before returning.  A server runs on the device that was current
+ * when it was created (else LUFQHE_ERROR_INVALID_VALUE). */
 #define LUFQHE_SERVE_MAX_INFLIGHT 8
 
 typedef struct lufqhe_server lufqhe_server;

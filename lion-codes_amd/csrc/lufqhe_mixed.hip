@@ -212,6 +212,11 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan* plan, const void* d_A, void* d_m
   const int src = source == LUFQHE_SOURCE_FQHE ? SRC_FQHE : SRC_MATRIX;
   if (plan->count == 0) return LUFQHE_SUCCESS;
   if (!d_A || !d_mant) return LUFQHE_ERROR_INVALID_VALUE;
+  // the plan's streams, events and tables belong to the device it was made on
+  int cur = 0;
+  hipError_t derr = hipGetDevice(&cur);
+  if (derr != hipSuccess) return hip_fail(derr);
+  if (cur != plan->device) return LUFQHE_ERROR_INVALID_VALUE;
   const DetOut out{static_cast<double2*>(d_mant), d_exp, d_logabs, d_phase, d_info, nullptr};
   hipError_t err = hipSuccess;
   if (mode == LUFQHE_MIXED_PLAIN) {

@@ -1,5 +1,5 @@
 // lup_dispatch.hpp -- launch tables filled by the per-shard instantiation
-// units (lup_inst_uniform.hip compiled once per kernel variant and shard).
+// units (lup_inst.hip compiled once per kernel variant and shard).
 #pragma once
 
 #include <hip/hip_runtime.h>

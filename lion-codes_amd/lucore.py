@@ -161,7 +161,7 @@ def shape_info(n: int, shape: int) -> tuple[int, int, int]:
 
 
 def set_shape(policy: str, arith: str, n: int, shape: int) -> bool:
-    return _lib.lufqhe_set_shape(POLICIES[policy], ARITHS[arith], n, shape) == SUCCESS
+    return load().lufqhe_set_shape(POLICIES[policy], ARITHS[arith], n, shape) == SUCCESS
 
 
 def get_shape(policy: str, arith: str, n: int) -> int:
@@ -170,6 +170,24 @@ def get_shape(policy: str, arith: str, n: int) -> int:
 
 def _ptr(t: torch.Tensor | None) -> int | None:
     return None if t is None else t.data_ptr()
+
+
+def _check_device_tensor(name: str, t: torch.Tensor, dtype, numel: int, device) -> None:
+    """A kernel writes or reads `numel` elements of `t`: refuse anything that
+    would put that access out of bounds or on another device."""
+    if t.dtype != dtype or not t.is_cuda or t.device != device:
+        raise ValueError(f"{name} must be a {dtype} GPU tensor on {device}")
+    if not t.is_contiguous() or t.numel() < numel:
+        raise ValueError(f"{name} must be contiguous with at least {numel} elements")
+
+
+def _check_workspace(ws: torch.Tensor | None, nbytes: int, device) -> None:
+    if ws is None:
+        return
+    if not ws.is_cuda or ws.device != device or not ws.is_contiguous():
+        raise ValueError(f"workspace must be a contiguous tensor on {device}")
+    if ws.numel() * ws.element_size() < nbytes:
+        raise ValueError(f"workspace must hold at least {nbytes} bytes")
 
 
 def _stream_handle(stream: torch.cuda.Stream | None) -> int:
@@ -192,6 +210,19 @@ class DetResult:
         """Plain complex determinant (overflows to inf where the reference's
         plain product would)."""
         return torch.ldexp(self.mant, self.exp.to(torch.float64))
+
+    def check(self, batch: int, n: int, device) -> None:
+        """Raise unless every output holds `batch` results on `device` (mant is
+        required; the library skips an output given as None)."""
+        if self.mant is None:
+            raise ValueError("mant is required")
+        fields = (("mant", torch.complex128, 1), ("exp", torch.int32, 1),
+                  ("logabs", torch.float64, 1), ("phase", torch.float64, 1),
+                  ("info", torch.int32, 1), ("piv", torch.int8, n))
+        for name, dtype, per in fields:
+            t = getattr(self, name)
+            if t is not None:
+                _check_device_tensor(name, t, dtype, batch * per, device)
 
 
 def alloc_result(batch: int, n: int, device, pivots: bool = False) -> DetResult:
@@ -216,6 +247,7 @@ def det_batched(A: torch.Tensor, policy: str = "crout_ref", arith: str = "fused"
     if n != n2 or A.stride(2) != 1:
         raise ValueError("A must hold square matrices with unit column stride")
     res = out if out is not None else alloc_result(B, n, A.device, pivots)
+    res.check(B, n, A.device)
     st = load().lufqhe_det_batched(
         _ptr(A), B, n, A.stride(1), A.stride(0), _ptr(res.mant), _ptr(res.exp),
         _ptr(res.logabs), _ptr(res.phase), _ptr(res.info), _ptr(res.piv),
@@ -233,6 +265,7 @@ def det_inplace(A: torch.Tensor, pair_product: bool = True, policy: str = "crout
     if A.dtype != torch.complex128 or not A.is_cuda or A.dim() != 3 or A.stride(2) != 1:
         raise ValueError("A must be a complex128 CUDA tensor [B, n, n] with unit column stride")
     B, n, _ = A.shape
+    _check_workspace(workspace, load().lufqhe_det_inplace_workspace_bytes(B), A.device)
     st = load().lufqhe_det_inplace(_ptr(A), B, n, A.stride(1), A.stride(0), int(pair_product),
                                    POLICIES[policy], _ptr(workspace), _stream_handle(stream))
     if st != SUCCESS:
@@ -242,6 +275,9 @@ def det_inplace(A: torch.Tensor, pair_product: bool = True, policy: str = "crout
 def global_reduction(A: torch.Tensor, workspace: torch.Tensor | None = None,
                      stream: torch.cuda.Stream | None = None) -> torch.Tensor:
     """sum_i A[2i,0,0] as one complex128 on the device (globalReduction)."""
+    if A.dtype != torch.complex128 or not A.is_cuda or A.dim() != 3:
+        raise ValueError("A must be a complex128 CUDA tensor of shape [B, n, n]")
+    _check_workspace(workspace, load().lufqhe_pair_sum_workspace_bytes(), A.device)
     out = torch.empty(1, dtype=torch.complex128, device=A.device)
     st = load().lufqhe_global_reduction(_ptr(A), A.shape[0], A.stride(0), _ptr(workspace),
                                         _ptr(out), _stream_handle(stream))
@@ -253,6 +289,10 @@ def global_reduction(A: torch.Tensor, workspace: torch.Tensor | None = None,
 def pair_sum(res: DetResult, workspace: torch.Tensor | None = None,
              stream: torch.cuda.Stream | None = None) -> tuple[torch.Tensor, torch.Tensor]:
     """sum_i det_2i * det_2i+1 = mant * 2**exp, from a DetResult (overflow free)."""
+    dev = res.mant.device
+    _check_device_tensor("mant", res.mant, torch.complex128, res.mant.numel(), dev)
+    _check_device_tensor("exp", res.exp, torch.int32, res.mant.numel(), dev)
+    _check_workspace(workspace, load().lufqhe_pair_sum_workspace_bytes(), dev)
     mant = torch.empty(1, dtype=torch.complex128, device=res.mant.device)
     exp = torch.empty(1, dtype=torch.int32, device=res.mant.device)
     st = load().lufqhe_pair_sum(_ptr(res.mant), _ptr(res.exp), res.mant.shape[0],
@@ -285,7 +325,10 @@ def fqhe_det(z: torch.Tensor, row_perm: torch.Tensor | None = None, count: int |
         raise ValueError("count must equal z.shape[0] unless z holds one configuration")
     if row_perm is not None and tuple(row_perm.shape) != (count, n):
         raise ValueError("row_perm must have shape [count, n]")
+    if row_perm is not None and row_perm.device != z.device:
+        raise ValueError("row_perm must be on the device of z")
     res = out if out is not None else alloc_result(count, n, z.device, pivots)
+    res.check(count, n, z.device)
     st = load().lufqhe_fqhe_det(_ptr(z), count, n, z_stride, _ptr(row_perm), _ptr(res.mant),
                                 _ptr(res.exp), _ptr(res.logabs), _ptr(res.phase),
                                 _ptr(res.info), _ptr(res.piv), POLICIES[policy], ARITHS[arith],
@@ -306,6 +349,8 @@ def fqhe_build(z: torch.Tensor, row_perm: torch.Tensor | None = None, count: int
         raise ValueError("count must equal z.shape[0] unless z holds one configuration")
     if row_perm is not None and tuple(row_perm.shape) != (count, n):
         raise ValueError("row_perm must have shape [count, n]")
+    if row_perm is not None and row_perm.device != z.device:
+        raise ValueError("row_perm must be on the device of z")
     A = torch.empty(count, n, n, dtype=torch.complex128, device=z.device)
     st = load().lufqhe_fqhe_build(_ptr(z), count, n, z_stride, _ptr(row_perm), _ptr(A),
                                   _stream_handle(stream))
@@ -454,6 +499,11 @@ class MixedPlan:
         self.ns = np.ascontiguousarray(ns, dtype=np.int32)
         self.offs = np.ascontiguousarray(offs, dtype=np.int64)
         self.count = len(self.ns)
+        if self.offs.shape != self.ns.shape:
+            raise ValueError("ns and offs must have the same length")
+        # complex elements of the flat buffer the plan reads, per source
+        self._need = {"matrix": int((self.offs + self.ns.astype(np.int64) ** 2).max(initial=0)),
+                      "fqhe": int((self.offs + self.ns).max(initial=0))}
         self._h = _vp()
         st = load().lufqhe_mixed_plan_create(self.ns.ctypes.data, self.offs.ctypes.data,
                                              self.count, POLICIES[policy], nstreams,
@@ -475,8 +525,12 @@ class MixedPlan:
                 stream: torch.cuda.Stream | None = None, source: str = "matrix") -> DetResult:
         """source "matrix": matrix m at flat[offs[m]:]; "fqhe": its ns[m]
         particle positions there, the FQHE matrix built on the device."""
-        if flat.dtype != torch.complex128 or not flat.is_cuda:
-            raise ValueError("flat must be a complex128 CUDA tensor")
+        if flat.dtype != torch.complex128 or not flat.is_cuda or not flat.is_contiguous():
+            raise ValueError("flat must be a contiguous complex128 CUDA tensor")
+        if flat.numel() < self._need[source]:
+            raise ValueError(f"flat holds {flat.numel()} elements; the plan reads "
+                             f"{self._need[source]} for source {source!r}")
+        out.check(self.count, 1, flat.device)
         st = _lib.lufqhe_mixed_execute(self._h, _ptr(flat), _ptr(out.mant), _ptr(out.exp),
                                        _ptr(out.logabs), _ptr(out.phase), _ptr(out.info),
                                        SOURCES[source], MIXED_MODES[mode],

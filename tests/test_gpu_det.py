@@ -135,6 +135,54 @@ def test_nullable_outputs(lc, dev, wl):
     assert torch.equal(mant, full.mant)
 
 
+def test_binding_refuses_short_buffers(lc, dev, wl):
+    """The Python binding checks every buffer a launch would touch before the
+    launch: short or misplaced outputs and workspaces raise, nothing runs."""
+    A = torch.from_numpy(wl.gaussian_np(8, 5, seed=2)).to(dev)
+    short = lc.alloc_result(7, 5, dev)
+    with pytest.raises(ValueError):
+        lc.det_batched(A, out=short)
+    no_piv_room = lc.alloc_result(8, 4, dev, pivots=True)
+    with pytest.raises(ValueError):
+        lc.det_batched(A, out=no_piv_room)
+    on_host = lc.alloc_result(8, 5, "cpu")
+    with pytest.raises(ValueError):
+        lc.det_batched(A, out=on_host)
+    ws = torch.empty(8 * 20 - 1, dtype=torch.uint8, device=dev)
+    with pytest.raises(ValueError):
+        lc.det_inplace(A.clone(), workspace=ws)
+    # outputs left as None are skipped by the library
+    r = lc.alloc_result(8, 5, dev)
+    r.logabs = r.phase = r.info = None
+    lc.det_batched(A, out=r)
+    torch.cuda.synchronize()
+    assert torch.equal(r.mant, lc.det_batched(A).mant)
+
+
+def test_mixed_plan_refuses_short_flat_buffer(lc, dev):
+    ns = np.array([3, 4], np.int32)
+    offs = np.array([0, 9], np.int64)
+    plan = lc.MixedPlan(ns, offs, nstreams=1)
+    try:
+        res = lc.alloc_result(2, 1, dev)
+        with pytest.raises(ValueError):
+            plan.execute(torch.zeros(24, dtype=torch.complex128, device=dev), res, mode="plain")
+        with pytest.raises(ValueError):
+            plan.execute(torch.zeros(25, dtype=torch.complex128, device=dev),
+                         lc.alloc_result(1, 1, dev), mode="plain")
+        # positions need only offs + n elements
+        plan.execute(torch.ones(13, dtype=torch.complex128, device=dev), res, mode="plain",
+                     source="fqhe")
+        flat = torch.zeros(25, dtype=torch.complex128, device=dev)
+        flat[0] = flat[4] = flat[8] = 1.0
+        flat[9] = flat[14] = flat[19] = flat[24] = 2.0
+        plan.execute(flat, res, mode="plain")
+        torch.cuda.synchronize()
+        assert res.det().cpu().numpy().tolist() == [1.0, 16.0]
+    finally:
+        plan.close()
+
+
 @pytest.mark.parametrize("n,count", [(16, 4096), (32, 65536)])
 def test_full_size_uniform_configs(lc, orc, wl, dev, n, count):
     """The 4096 x 16x16 and 65536 x 32x32 batches at full size: every status
