@@ -6,10 +6,14 @@ A step is one pass of the hot path over one batch: one plain launch of the
 n=32 kernel over 65536 complex128 matrices (re,im ~ N(0,1), seeded), inputs
 resident in HBM, outputs mant/exp/log|det|/phase/info per matrix.  Before the
 W warm-up steps the kernel runs untimed for DEVICE_WARM_S so that the GPU's
-clocks have ramped up.  Prints one
-JSON line (metric, value = det/s, roofline of the kernel measured with HIP
-events on its stream, CPU baseline of the restated reference Crout timed on
-this host's cores, accuracy against the long-double truth).
+clocks have ramped up.  Prints one JSON line: metric, value = det/s, the
+roofline of the kernel measured with HIP events on its stream, the other
+GPU measurements, and the CPU baseline leg.
+
+The CPU baseline leg runs last and is the only part that uses the oracle:
+it times the restated reference Crout on this host's cores and, as the
+checker, compares samples of every GPU leg's results with the oracle
+(accuracy against the long-double truth, agreement with the Crout).
 """
 from __future__ import annotations
 
@@ -88,19 +92,30 @@ def cpu_baseline(A_host: np.ndarray, cores: int) -> dict:
             "cpu_model": cpu_model()}
 
 
-def accuracy(lc, A, res, count: int = 512) -> dict:
-    """|det/det_truth - 1| against the long-double oracle on a sample."""
-    import torch
+def cpu_checks(out: dict, samples: dict) -> None:
+    """The checker half of the CPU baseline leg: the GPU results sampled by
+    each leg against the oracle, written into that leg's entry of `out`."""
     from oracle import oracle
-    idx = torch.arange(0, BATCH, BATCH // count, device=A.device)[:count]
-    As = A[idx].cpu().numpy()
-    t = oracle.truth(As)
-    la = res.logabs[idx].cpu().numpy()
-    ph = res.phase[idx].cpu().numpy()
-    e = np.abs(np.expm1((la - t["logabs"]) + 1j * np.angle(np.exp(1j * (ph - t["phase"])))))
-    return {"vs": "long-double LUP truth", "sample": int(count), "median": float(np.median(e)),
-            "p99": float(np.quantile(e, 0.99)), "max": float(e.max()),
-            "frac_le_1e-12": float(np.mean(e <= 1e-12))}
+    if "uniform" in samples:
+        A_s, la, ph = samples["uniform"]
+        t = oracle.truth(A_s)
+        e = log_err_vs(la, ph, t["logabs"], t["phase"])
+        out["accuracy"] = {"vs": "long-double LUP truth", "sample": int(len(A_s)),
+                           "median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
+                           "max": float(e.max()), "frac_le_1e-12": float(np.mean(e <= 1e-12))}
+    if "small" in samples:
+        A_s, det = samples["small"]
+        o = oracle.crout_ref(A_s)
+        out["small_batch_n16"]["vs_crout_ref_max_rel"] = float(np.abs(det / o["det"] - 1.0).max())
+    if "mixed" in samples:
+        sub, sub_ns, sub_offs, la, ph, well = samples["mixed"]
+        t = oracle.truth_mixed(sub, sub_ns, sub_offs)
+        e = log_err_vs(la, ph, t["logabs"], t["phase"])
+        out["mixed"]["accuracy"] = {
+            "vs": "long-double LUP truth", "sample": len(sub_ns),
+            "median": float(np.median(e[well])), "p99": float(np.quantile(e[well], 0.99)),
+            "frac_le_1e-12": float(np.mean(e[well] <= 1e-12)),
+            "near_singular_median": float(np.median(e[~well])) if (~well).any() else None}
 
 
 def time_events(fn, stream, reps: int) -> float:
@@ -141,28 +156,24 @@ def bench_h2d(lc, A, res, reps: int) -> dict:
             "note": "PCIe-inclusive rate of the n=32 batch; not the headline value"}
 
 
-def bench_small_batch(lc, wl, dev, reps: int) -> dict:
+def bench_small_batch(lc, wl, dev, reps: int, samples: dict) -> dict:
     """4096 complex128 16x16 matrices -- a batch small enough that
     one launch is latency-bound; reported as launch time and det/s."""
     import torch
-    from oracle import oracle
     A = wl.gaussian(4096, 16, 11, device=dev)
     res = lc.alloc_result(4096, 16, dev)
     s = torch.cuda.current_stream()
     lc.det_batched(A, out=res)
     ms = time_events(lambda: lc.det_batched(A, out=res), s, reps)
-    o = oracle.crout_ref(A.cpu().numpy())
-    rel = np.abs(res.det().cpu().numpy() / o["det"] - 1.0)
+    samples["small"] = (A.cpu().numpy(), res.det().cpu().numpy())
     return {"workload": "4096 complex128 16x16, re,im ~ N(0,1), seed 11, one launch",
-            "ms": ms, "det_per_s": 4096 / (ms * 1e-3),
-            "vs_crout_ref_max_rel": float(rel.max())}
+            "ms": ms, "det_per_s": 4096 / (ms * 1e-3)}
 
 
-def bench_mixed(lc, wl, dev, reps: int) -> dict:
+def bench_mixed(lc, wl, dev, reps: int, samples: dict) -> dict:
     """Mixed-size batch: 200000 matrices, n ~ U{8..64}, 5% near-singular; one
     hipGraph replay of the bucketed plan vs per-size plain launches."""
     import torch
-    from oracle import oracle
     count, seed = 200000, 2024
     flat, ns, offs, sing = wl.mixed_batch(count, seed, device=dev)
     plan = lc.MixedPlan(ns, offs, nstreams=4)
@@ -199,21 +210,15 @@ def bench_mixed(lc, wl, dev, reps: int) -> dict:
     out["speedup_graph_vs_plain"] = out["plain"]["ms"] / out["graph"]["ms"]
     out["speedup_graph_vs_repacked"] = out["repacked_per_size"]["ms"] / out["graph"]["ms"]
     assert int((res.info != 0).sum()) == 0
-    # accuracy on a sample against the long-double truth, by matrix class
+    # a sample for the CPU leg's accuracy check, by matrix class
     rng = np.random.default_rng(0)
     idx = np.sort(rng.choice(count, 1000, replace=False))
     sub_ns = ns[idx]
     sub_offs = np.zeros(len(idx), np.int64)
     sub_offs[1:] = np.cumsum(sub_ns.astype(np.int64) ** 2)[:-1]
     sub = torch.cat([flat[int(offs[i]):int(offs[i]) + int(ns[i]) ** 2] for i in idx]).cpu().numpy()
-    t = oracle.truth_mixed(sub, sub_ns, sub_offs)
-    e = log_err_vs(res.logabs.cpu().numpy()[idx], res.phase.cpu().numpy()[idx],
-                   t["logabs"], t["phase"])
-    well = ~sing[idx]
-    out["accuracy"] = {"vs": "long-double LUP truth", "sample": len(idx),
-                       "median": float(np.median(e[well])), "p99": float(np.quantile(e[well], 0.99)),
-                       "frac_le_1e-12": float(np.mean(e[well] <= 1e-12)),
-                       "near_singular_median": float(np.median(e[~well])) if (~well).any() else None}
+    samples["mixed"] = (sub, sub_ns, sub_offs, res.logabs.cpu().numpy()[idx],
+                        res.phase.cpu().numpy()[idx], ~sing[idx])
     plan.close()
     del flat
     torch.cuda.empty_cache()
@@ -382,12 +387,13 @@ def main() -> None:
                      "fp64_frac": flops_launch / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                      "bytes_per_det": det_bytes(N), "flops_per_det": det_flops(N)},
     }
-    out["accuracy"] = accuracy(lc, A, res)
-    out["small_batch_n16"] = bench_small_batch(lc, wl, dev, max(5, args.steps))
+    samples = {}
+    acc_idx = torch.arange(0, BATCH, BATCH // 512, device=dev)[:512]
+    samples["uniform"] = (A[acc_idx].cpu().numpy(), res.logabs[acc_idx].cpu().numpy(),
+                          res.phase[acc_idx].cpu().numpy())
+    out["small_batch_n16"] = bench_small_batch(lc, wl, dev, max(5, args.steps), samples)
     out["h2d_inclusive"] = bench_h2d(lc, A, res, 5)
-    if not args.no_cpu:
-        A_host = A[:16384].cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(A_host, host_cores())
+    A_host = None if args.no_cpu else A[:16384].cpu().numpy()
     legs = [] if args.no_extra else [x for x in args.legs.split(",") if x]
     if legs:
         del A
@@ -395,13 +401,17 @@ def main() -> None:
         reps = max(3, min(args.steps, 10))
     if "mixed" in legs:
         print("# mixed-size batch", file=sys.stderr, flush=True)
-        out["mixed"] = bench_mixed(lc, wl, dev, reps)
+        out["mixed"] = bench_mixed(lc, wl, dev, reps, samples)
     if "fqhe" in legs:
         print("# FQHE batch", file=sys.stderr, flush=True)
         out["fqhe"] = bench_fqhe(lc, wl, dev, reps)
     if "serving" in legs:
         print("# walker serving", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(lc, dev)
+    if not args.no_cpu:
+        print("# CPU baseline and checks", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(A_host, host_cores())
+        cpu_checks(out, samples)
     print(json.dumps(out), flush=True)
 
 
