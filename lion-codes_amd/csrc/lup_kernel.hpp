@@ -540,7 +540,9 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
     if (k + 1 < N) {
       double ir = 0.0, ii = 0.0;
       if constexpr (ARITH == ARITH_FUSED) {
-        const double rden = 1.0 / (zero ? 1.0 : den);
+        // a zero pivot (|p|^2 == 0, also for a nonzero p whose square
+        // underflows) gets exactly zero multipliers: the step changes no row
+        const double rden = zero ? 0.0 : 1.0 / den;
         ir = p.x * rden;
         ii = -(p.y * rden);
       }
@@ -575,6 +577,8 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
       // multipliers are exactly zero (ir = ii = 0 in fused arithmetic, an
       // explicit zero in reference arithmetic), and fma(-0, u, e) == e for
       // finite u up to the sign of a zero element, which no output sees.
+      // (An infinite entry in such a step's pivot row or column turns into
+      // NaN here, where the reference-order elimination skips the step.)
       ppos = update_and_search<L, CG, POLICY, ARITH>(a, lx, ly, bc, cb, k, pos, act, sc, rs);
     }
     wave_lds_sync();

@@ -135,6 +135,67 @@ def test_nullable_outputs(lc, dev, wl):
     assert torch.equal(mant, full.mant)
 
 
+def same_or_both_nan(g, o):
+    """Element-wise: equal bits, or NaN on both sides (a NaN the hardware
+    generates carries its own sign bit: +NaN on the GPU, -NaN on x86)."""
+    g = np.ascontiguousarray(g).view(np.float64)
+    o = np.ascontiguousarray(o).view(np.float64)
+    return (g.view(np.uint64) == o.view(np.uint64)) | (np.isnan(g) & np.isnan(o))
+
+
+@pytest.mark.parametrize("n", [5, 16, 32, 48, 64])
+def test_extreme_and_non_finite_inputs(lc, orc, wl, dev, n):
+    """Entries near the ends of the FP64 range and non-finite entries take
+    the mirror's path: rows below the row-scale floor are rejected, |p|^2
+    overflowing to inf gives zero multipliers, NaN criteria count as DBL_MAX,
+    subnormals are kept. Status, exponent and pivots match exactly, the
+    mantissa bit for bit or as NaN on both sides."""
+    base = wl.gaussian_np(12, n, seed=300 + n)
+    A = np.empty_like(base)
+    # exact power-of-two scalings: tiny (rejected), large, |p|^2 near and past overflow
+    for m, k in enumerate((-90, 60, 400, 508, 515, 600)):
+        A[m] = base[m] * 2.0 ** k
+    A[6] = base[6]
+    A[6][1 % n, 2 % n] = np.inf                       # one infinite real part
+    A[7] = base[7]
+    A[7][0, 0] = complex(np.nan, 0.0)                  # NaN on the first pivot's row
+    A[8] = base[8]
+    A[8][n - 1, :] = complex(np.nan, np.nan)           # a whole NaN row
+    A[9] = base[9]
+    A[9][n // 2, 0] = complex(1.0, np.inf)             # infinite imaginary part
+    A[10] = base[10]
+    A[10][::2, :] *= 1e-310 / 2 ** -1022 * 2.0 ** -1030  # rows of subnormal magnitude
+    A[11] = base[11] * 1e-12                           # small but above the row floor
+    for policy in ("crout_ref", "abs"):
+        g = run(lc, A, dev, policy, "fused")
+        o = orc.mirror(A, policy, "fused")
+        assert np.array_equal(g["info"], o["info"]), (policy, g["info"], o["info"])
+        assert np.array_equal(g["exp"], o["exp"]), (policy, g["exp"], o["exp"])
+        assert same_or_both_nan(g["mant"], o["mant"]).all(), policy
+        ok = g["info"] >= 0
+        assert np.array_equal(g["piv"][ok], o["piv"][ok]), policy
+
+
+def test_zero_pivot_with_underflowed_modulus(lc, orc, dev):
+    """|p|^2 underflows to 0 although p != 0: a zero pivot (status k+1), and
+    the step must leave the other rows untouched, as the reference-order
+    mirror does. Row 1's scale (1e150) keeps it from being chosen while its
+    multiplier a_10 conj(p) = 1e-191 is large enough to change its 1e-180
+    entry, which later becomes the last pivot (det = 2e-200)."""
+    A = np.zeros((2, 3, 3), np.complex128)
+    A[0] = [[1e-170, 0.0, 1.0],
+            [1e-21, 1e150, 1e-180],
+            [0.0, 1e150, 3e-180]]
+    A[1] = A[0] * (0.6 + 0.8j)
+    for policy in ("crout_ref", "abs"):
+        g = run(lc, A, dev, policy, "fused")
+        o = orc.mirror(A, policy, "fused")
+        assert_same(g, o, 3, policy)
+    g = run(lc, A, dev)
+    assert g["info"].tolist() == [1, 1]
+    assert abs(g["mant"][0] * 2.0 ** int(g["exp"][0]) / 2e-200 - 1) < 1e-12
+
+
 def test_binding_refuses_short_buffers(lc, dev, wl):
     """The Python binding checks every buffer a launch would touch before the
     launch: short or misplaced outputs and workspaces raise, nothing runs."""
