@@ -40,7 +40,7 @@ OUT_BYTES = 16 + 4 + 8 + 8 + 4   # mant, exp, logabs, phase, info
 # HBM bytes one launch of the n=32 kernel moves, measured with rocprofv3
 # (FETCH_SIZE and WRITE_SIZE, one counter per run, each scaled by what the
 # same counter reads for a plain 1 GiB device copy: x2.0 and x1.0).
-MEASURED_TRAFFIC_N32 = 1.0800e9
+MEASURED_TRAFFIC_N32 = 1.0836e9
 DEVICE_WARM_S = 0.25  # untimed device warm-up before the W warm-up steps
 
 
