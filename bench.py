@@ -172,7 +172,8 @@ def bench_small_batch(lc, wl, dev, reps: int, samples: dict) -> dict:
 
 def bench_mixed(lc, wl, dev, reps: int, samples: dict) -> dict:
     """Mixed-size batch: 200000 matrices, n ~ U{8..64}, 5% near-singular; one
-    hipGraph replay of the bucketed plan vs per-size plain launches."""
+    hipGraph replay of the bucketed plan (per-size launches over streams, or
+    the persistent class kernels) vs per-size plain launches."""
     import torch
     count, seed = 200000, 2024
     flat, ns, offs, sing = wl.mixed_batch(count, seed, device=dev)
@@ -185,7 +186,7 @@ def bench_mixed(lc, wl, dev, reps: int, samples: dict) -> dict:
     total_bytes = float(np.sum(16.0 * ns.astype(np.float64) ** 2 + OUT_BYTES))
     total_flops = float(np.sum([det_flops(int(n)) for n in ns]))
     with torch.cuda.stream(side):
-        for mode in ("plain", "streams", "graph"):
+        for mode in ("plain", "streams", "graph", "persistent"):
             plan.execute(flat, res, mode=mode, stream=side)   # warm-up / capture
             ms = time_events(lambda: plan.execute(flat, res, mode=mode, stream=side), side, reps)
             out[mode] = {"ms": ms, "det_per_s": count / (ms * 1e-3),
@@ -208,6 +209,7 @@ def bench_mixed(lc, wl, dev, reps: int, samples: dict) -> dict:
         out["repacked_per_size"] = {"ms": ms, "det_per_s": count / (ms * 1e-3)}
     side.synchronize()
     out["speedup_graph_vs_plain"] = out["plain"]["ms"] / out["graph"]["ms"]
+    out["speedup_persistent_vs_plain"] = out["plain"]["ms"] / out["persistent"]["ms"]
     out["speedup_graph_vs_repacked"] = out["repacked_per_size"]["ms"] / out["graph"]["ms"]
     assert int((res.info != 0).sum()) == 0
     # a sample for the CPU leg's accuracy check, by matrix class

@@ -185,12 +185,22 @@ int32_t lufqhe_fqhe_build(const void *d_z, int64_t count, int32_t n,
  *                         be the legacy default stream (capture is not
  *                         allowed there);
  *   LUFQHE_MIXED_STREAMS  the same forked launches, issued directly;
- *   LUFQHE_MIXED_PLAIN    one launch per size, one after another on `stream`.
+ *   LUFQHE_MIXED_PLAIN    one launch per size, one after another on `stream`;
+ *   LUFQHE_MIXED_PERSISTENT  one persistent kernel per size class (n <= 16,
+ *                         17..32, 33..48, 49..57, 58..64) instead of one
+ *                         launch per bucket: resident waves drain the class's
+ *                         buckets, largest n first, from a device work
+ *                         counter that a kernel at the head of the graph
+ *                         zeroes.  Captured and replayed like GRAPH (the
+ *                         per-bucket arguments are written to device memory,
+ *                         synchronously, when the pointers change).  Fused
+ *                         arithmetic with either policy.
  * A plan is not re-entrant: one execute at a time per plan, on the device
  * that was current when it was created (else LUFQHE_ERROR_INVALID_VALUE). */
 #define LUFQHE_MIXED_GRAPH 0
 #define LUFQHE_MIXED_STREAMS 1
 #define LUFQHE_MIXED_PLAIN 2
+#define LUFQHE_MIXED_PERSISTENT 3
 #define LUFQHE_MIXED_MAX_STREAMS 8
 #define LUFQHE_SOURCE_MATRIX 0
 #define LUFQHE_SOURCE_FQHE 1

@@ -9,7 +9,8 @@
   library never links it.
 
 Kernels are instantiated per (variant, n); ``lup_inst.hip`` is compiled once
-per (variant, shard) so the fully unrolled kernels compile in parallel.
+per (variant, shard) so the fully unrolled kernels compile in parallel, and
+``lup_persist_inst.hip`` once per (fused variant, size class).
 """
 from __future__ import annotations
 
@@ -31,6 +32,8 @@ ORACLE_LIB = ORACLE_DIR / "liblufqhe_oracle.so"
 ARCH = "gfx950"
 N_VARIANTS = 3
 N_SHARDS = 8
+N_PERSIST_VARIANTS = 2  # persistent mixed-size kernels: fused CROUT_REF and ABS
+N_PERSIST_CLASSES = 5   # size classes of lup_persistent.hpp
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -75,13 +78,20 @@ def _run(cmd: list[str], verbose: bool) -> None:
 def _compile_jobs() -> list[tuple[Path, list[str], list[Path]]]:
     hdrs = _headers()
     jobs = []
+    # the persistent class kernels are the longest single compiles: first
+    for v in range(N_PERSIST_VARIANTS):
+        for c in range(N_PERSIST_CLASSES):
+            obj = BUILD / f"lup_persist_v{v}_c{c}.o"
+            defs = [f"-DLUFQHE_VARIANT={v}", f"-DLUFQHE_CLASS={c}"]
+            jobs.append((obj, defs + ["-c", str(CSRC / "lup_persist_inst.hip")],
+                         hdrs + [CSRC / "lup_persist_inst.hip"]))
     for v in range(N_VARIANTS):
         for s in range(N_SHARDS):
             obj = BUILD / f"lup_inst_v{v}_s{s}.o"
             defs = [f"-DLUFQHE_VARIANT={v}", f"-DLUFQHE_SHARD={s}", f"-DLUFQHE_SHARDS={N_SHARDS}"]
             jobs.append((obj, defs + ["-c", str(CSRC / "lup_inst.hip")], hdrs + [CSRC / "lup_inst.hip"]))
     for src in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp")):
-        if src.name == "lup_inst.hip":
+        if src.name in ("lup_inst.hip", "lup_persist_inst.hip"):
             continue
         obj = BUILD / (src.stem + ".o")
         lang = ["-x", "hip"] if src.suffix == ".hip" else []

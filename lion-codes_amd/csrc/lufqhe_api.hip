@@ -8,8 +8,9 @@
 
 namespace lufqhe {
 
-// Filled at load time by the lup_inst.hip objects.
+// Filled at load time by the lup_inst.hip and lup_persist_inst.hip objects.
 KernelEntry g_kernels[kNumVariants][kMaxN + 1][kMaxShapes];
+PersistEntry g_persist[kPersistVariants][kPersistClasses];
 
 namespace {
 thread_local hipError_t t_last_hip = hipSuccess;

@@ -13,6 +13,10 @@
 //     beside the big ones instead of after them;
 //   * the whole fork/launch/join sequence is captured once into a hipGraph
 //     and replayed: one graph launch per batch.
+// Mode PERSISTENT replaces the per-bucket launches by one persistent kernel
+// per size class (lup_persistent.hpp): the buckets of a class form one
+// device work queue, drained largest n first by resident waves; the queue
+// counters are zeroed by a small kernel at the head of the captured graph.
 // Mode PLAIN runs the same kernels one size after another on the caller's
 // stream: the uniform-launch-per-size form it is compared with.
 #include <hip/hip_runtime.h>
@@ -49,11 +53,32 @@ struct lufqhe_mixed_plan {
   hipStream_t streams[LUFQHE_MIXED_MAX_STREAMS] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[LUFQHE_MIXED_MAX_STREAMS] = {};
-  // captured graph and the pointers it was captured for
+  // persistent mode: per class, its buckets' queue and launch size
+  struct PClass {
+    int cls = 0;
+    int64_t first = 0;  // first bucket of the class in the device arrays
+    int32_t nbuckets = 0;
+    int64_t total_tasks = 0;
+    int64_t blocks = 0;
+    double cost = 0.0;
+    int stream = 0;
+  };
+  std::vector<PClass> pclasses;           // in launch order
+  std::vector<BatchArgs> h_bargs;         // per bucket, in class order
+  std::vector<int32_t> h_bucket_index;    // bargs slot -> plan bucket
+  void* d_persist = nullptr;              // PersistBucket[] | BatchArgs[] | counters
+  PersistBucket* d_pbuckets = nullptr;
+  BatchArgs* d_bargs = nullptr;
+  uint32_t* d_counters = nullptr;
+  const void* p_A = nullptr;              // pointers d_bargs holds
+  int p_src = -1;
+  DetOut p_out{};
+  // captured graph and what it was captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void* g_A = nullptr;
   int g_src = 0;
+  int g_mode = -1;
   DetOut g_out{};
 
   ~lufqhe_mixed_plan() {
@@ -65,6 +90,7 @@ struct lufqhe_mixed_plan {
     }
     if (fork) (void)hipEventDestroy(fork);
     if (d_tables) (void)hipFree(d_tables);
+    if (d_persist) (void)hipFree(d_persist);
   }
 };
 
@@ -110,6 +136,121 @@ hipError_t issue_forked(const lufqhe_mixed_plan& p, const void* d_A, int src,
 bool same_out(const DetOut& a, const DetOut& b) {
   return a.mant == b.mant && a.expo == b.expo && a.logabs == b.logabs && a.phase == b.phase &&
          a.info == b.info && a.piv == b.piv;
+}
+
+__global__ void __launch_bounds__(64) persist_reset_kernel(uint32_t* __restrict__ counters, int n) {
+  if ((int)threadIdx.x < n) counters[threadIdx.x] = 0u;
+}
+
+// The class queues of a plan: buckets grouped by size class, largest n
+// first within a class, with their task ranges; classes ordered and spread
+// over the plan streams by cost.  Device arrays are allocated here; the
+// per-bucket launch arguments are written when the pointers are known.
+hipError_t build_persistent(lufqhe_mixed_plan& p) {
+  if (p.variant >= kPersistVariants) return hipSuccess;  // no persistent kernels for it
+  int cus = 0;
+  hipError_t err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device);
+  if (err != hipSuccess) return err;
+  std::vector<PersistBucket> hb;
+  for (int c = 0; c < kPersistClasses; ++c) {
+    std::vector<int> idx;
+    for (int i = 0; i < (int)p.buckets.size(); ++i)
+      if (persist_class(p.buckets[i].n) == c) idx.push_back(i);
+    if (idx.empty()) continue;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return p.buckets[a].n > p.buckets[b].n; });
+    lufqhe_mixed_plan::PClass pc;
+    pc.cls = c;
+    pc.first = (int64_t)hb.size();
+    pc.nbuckets = (int32_t)idx.size();
+    for (int i : idx) {
+      const auto& b = p.buckets[i];
+      const int g = persist_group(b.n);
+      hb.push_back(PersistBucket{b.n, 0, pc.total_tasks});
+      p.h_bucket_index.push_back(i);
+      pc.total_tasks += (b.count + g - 1) / g;
+      pc.cost += b.cost;
+    }
+    const PersistEntry& e = g_persist[p.variant][c];
+    if (!e.launch) return hipErrorInvalidDeviceFunction;
+    int per_cu = 0;
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, e.kernel, 64, 0);
+    if (err != hipSuccess) return err;
+    pc.blocks = std::min<int64_t>(pc.total_tasks, (int64_t)std::max(1, per_cu) * cus);
+    p.pclasses.push_back(pc);
+  }
+  std::sort(p.pclasses.begin(), p.pclasses.end(),
+            [](const auto& a, const auto& b) { return a.cost > b.cost; });
+  std::vector<double> load(p.nstreams, 0.0);
+  for (auto& pc : p.pclasses) {
+    const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    pc.stream = s;
+    load[s] += pc.cost;
+  }
+  p.h_bargs.assign(hb.size(), BatchArgs{});
+  const size_t nb = hb.size();
+  const size_t o_bargs = (nb * sizeof(PersistBucket) + 255) & ~(size_t)255;
+  const size_t o_cnt = (o_bargs + nb * sizeof(BatchArgs) + 255) & ~(size_t)255;
+  if (nb == 0) return hipSuccess;
+  err = hipMalloc(&p.d_persist, o_cnt + kPersistClasses * sizeof(uint32_t));
+  if (err != hipSuccess) return err;
+  char* base = static_cast<char*>(p.d_persist);
+  p.d_pbuckets = reinterpret_cast<PersistBucket*>(base);
+  p.d_bargs = reinterpret_cast<BatchArgs*>(base + o_bargs);
+  p.d_counters = reinterpret_cast<uint32_t*>(base + o_cnt);
+  return hipMemcpy(p.d_pbuckets, hb.data(), nb * sizeof(PersistBucket), hipMemcpyHostToDevice);
+}
+
+// Per-bucket launch arguments for these pointers (synchronous, outside any
+// capture; the caller has drained `stream`, the only user of the old ones).
+hipError_t write_persistent_args(lufqhe_mixed_plan& p, const void* d_A, int src,
+                                 const DetOut& out) {
+  for (size_t j = 0; j < p.h_bargs.size(); ++j) {
+    const auto& b = p.buckets[p.h_bucket_index[j]];
+    BatchArgs& a = p.h_bargs[j];
+    a = BatchArgs{};
+    a.A = static_cast<const double2*>(d_A);
+    a.count = b.count;
+    a.ld = b.n;
+    a.stride = 0;
+    a.offs = p.d_tables + b.start;
+    a.out_idx = p.d_tables + p.count + b.start;
+    a.piv_stride = 0;
+    a.src = src;
+    a.out = out;
+  }
+  hipError_t err = hipMemcpy(p.d_bargs, p.h_bargs.data(), p.h_bargs.size() * sizeof(BatchArgs),
+                             hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    p.p_A = d_A;
+    p.p_src = src;
+    p.p_out = out;
+  }
+  return err;
+}
+
+// Reset the class counters, fork, one persistent launch per class, join.
+hipError_t issue_persistent(const lufqhe_mixed_plan& p, hipStream_t stream) {
+  hipLaunchKernelGGL(persist_reset_kernel, dim3(1), dim3(64), 0, stream, p.d_counters,
+                     kPersistClasses);
+  hipError_t err = hipGetLastError();
+  if (err == hipSuccess) err = hipEventRecord(p.fork, stream);
+  for (int s = 0; s < p.nstreams && err == hipSuccess; ++s)
+    err = hipStreamWaitEvent(p.streams[s], p.fork, 0);
+  for (const auto& pc : p.pclasses) {
+    if (err != hipSuccess) break;
+    PersistArgs pa{};
+    pa.buckets = p.d_pbuckets + pc.first;
+    pa.bargs = p.d_bargs + pc.first;
+    pa.nbuckets = pc.nbuckets;
+    pa.counter = p.d_counters + pc.cls;
+    pa.total_tasks = pc.total_tasks;
+    err = g_persist[p.variant][pc.cls].launch(pa, pc.blocks, p.streams[pc.stream]);
+  }
+  for (int s = 0; s < p.nstreams && err == hipSuccess; ++s) {
+    err = hipEventRecord(p.join[s], p.streams[s]);
+    if (err == hipSuccess) err = hipStreamWaitEvent(stream, p.join[s], 0);
+  }
+  return err;
 }
 
 }  // namespace
@@ -182,6 +323,11 @@ int32_t lufqhe_mixed_plan_create(const int32_t* h_n, const int64_t* h_offs, int6
       delete p;
       return LUFQHE_ERROR_NOT_BUILT;
     }
+  err = build_persistent(*p);
+  if (err != hipSuccess) {
+    delete p;
+    return hip_fail(err);
+  }
   *out = p;
   return LUFQHE_SUCCESS;
 }
@@ -229,16 +375,26 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan* plan, const void* d_A, void* d_m
     }
   } else if (mode == LUFQHE_MIXED_STREAMS) {
     err = issue_forked(*plan, d_A, src, out, stream);
-  } else if (mode == LUFQHE_MIXED_GRAPH) {
+  } else if (mode == LUFQHE_MIXED_GRAPH || mode == LUFQHE_MIXED_PERSISTENT) {
     if (stream == nullptr) return LUFQHE_ERROR_INVALID_VALUE;  // no capture on stream 0
-    if (!plan->exec || plan->g_A != d_A || plan->g_src != src || !same_out(plan->g_out, out)) {
+    const bool persistent = mode == LUFQHE_MIXED_PERSISTENT;
+    if (persistent && plan->variant >= kPersistVariants) return LUFQHE_ERROR_UNSUPPORTED;
+    if (!plan->exec || plan->g_mode != mode || plan->g_A != d_A || plan->g_src != src ||
+        !same_out(plan->g_out, out)) {
       if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
       if (plan->graph) (void)hipGraphDestroy(plan->graph);
       plan->exec = nullptr;
       plan->graph = nullptr;
-      err = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+      plan->g_mode = -1;
+      if (persistent && (plan->p_A != d_A || plan->p_src != src || !same_out(plan->p_out, out))) {
+        // earlier replays on `stream` may still read the old arguments
+        err = hipStreamSynchronize(stream);
+        if (err == hipSuccess) err = write_persistent_args(*plan, d_A, src, out);
+      }
+      if (err == hipSuccess) err = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
       if (err == hipSuccess) {
-        const hipError_t e2 = issue_forked(*plan, d_A, src, out, stream);
+        const hipError_t e2 = persistent ? issue_persistent(*plan, stream)
+                                         : issue_forked(*plan, d_A, src, out, stream);
         err = hipStreamEndCapture(stream, &plan->graph);
         if (e2 != hipSuccess) err = e2;
       }
@@ -246,6 +402,7 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan* plan, const void* d_A, void* d_m
       if (err == hipSuccess) {
         plan->g_A = d_A;
         plan->g_src = src;
+        plan->g_mode = mode;
         plan->g_out = out;
       }
     }

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "lup_kernel.hpp"
+#include "lup_persistent.hpp"
 
 namespace lufqhe {
 
@@ -52,5 +53,15 @@ extern KernelEntry g_kernels[kNumVariants][kMaxN + 1][kMaxShapes];
 // Shape chosen per (variant, n); 0 until the autotuner's table is applied.
 int selected_shape(int variant, int n);
 void set_selected_shape(int variant, int n, int shape);
+
+// Persistent mixed-size kernels (lup_persistent.hpp), one per fused variant
+// (0: CROUT_REF, 1: ABS) and size class, filled by lup_persist_inst.hip.
+using PersistLaunch = hipError_t (*)(const PersistArgs&, int64_t blocks, hipStream_t stream);
+struct PersistEntry {
+  PersistLaunch launch;
+  const void* kernel;  // for occupancy queries
+};
+constexpr int kPersistVariants = 2;
+extern PersistEntry g_persist[kPersistVariants][kPersistClasses];
 
 }  // namespace lufqhe

@@ -654,87 +654,93 @@ __device__ __forceinline__ double2 cmul_ref(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
 }
 
-template <int N, int L, int CG, int POLICY, int ARITH, int WPB, int OCC>
-__global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) {
+// One task: the G matrices task*G .. task*G + G-1 of `args`, factored by one
+// wave whose LDS broadcast area is `bc_wave` (G slots of NPAD+1 elements).
+// The caller has checked that the task holds at least one matrix.
+template <int N, int L, int CG, int POLICY, int ARITH>
+__device__ __forceinline__ void det_task(const BatchArgs& args, int64_t task,
+                                         double2* __restrict__ bc_wave, int lane) {
   using T = Tile<N, L, CG>;
   constexpr int R = T::R, C = T::C, RG = T::RG, G = T::G;
   constexpr int NP = T::NPAD + 1;  // padded LDS slot: groups on distinct banks
   constexpr int PR = (N + L - 1) / L;
-  __shared__ double2 bcast[WPB * G * NP];
-
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int g = lane / L;
   const int gl = lane % L;
   const int rb = gl / CG, cb = gl % CG;
-  double2* bc = bcast + (wave * G + g) * NP;
+  double2* bc = bc_wave + g * NP;
+  const int64_t i = task * G + g;
+  const bool mat_ok = i < args.count;
+
+  double2 a[R][C];
+  const double2* __restrict__ base = nullptr;
+  if (mat_ok) base = args.offs ? args.A + args.offs[i] : args.A + i * args.stride;
+  const int64_t ld = args.offs ? (int64_t)N : args.ld;
+  const uint8_t* __restrict__ perm = nullptr;
+  if (mat_ok && args.row_perm)
+    perm = args.row_perm + (args.perm_offs ? args.perm_offs[i] : i * N);
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int row = rb + t * RG;
+    if (args.src == SRC_FQHE && mat_ok && row < N) {
+      // positions: matrix i's z at A + offs[i] (or i*stride), one per row
+      int src = row;
+      if (perm) src = min((int)perm[row], N - 1);
+      const double2 z = base[src];
+      const double g = fqhe_gauss(z);
+      double2 pw = make_double2(1.0, 0.0);
+#pragma unroll
+      for (int s = 0; s < C; ++s) a[t][s] = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        if (j % CG == cb) a[t][j / CG] = make_double2(pw.x * g, pw.y * g);
+        pw = cmul_ref(pw, z);
+      }
+    } else if (mat_ok && row < N) {
+      int src = row;
+      if (perm) src = min((int)perm[row], N - 1);
+      const double2* __restrict__ rp = base + (int64_t)src * ld;
+#pragma unroll
+      for (int s = 0; s < C; ++s) {
+        const int col = cb + s * CG;
+        a[t][s] = (col < N) ? rp[col] : make_double2(0.0, 0.0);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < C; ++s)
+        a[t][s] = make_double2((cb + s * CG) == row ? 1.0 : 0.0, 0.0);
+    }
+  }
+
+  int piv_rec[PR];
+#pragma unroll
+  for (int q = 0; q < PR; ++q) piv_rec[q] = -1;
+  const GroupResult res = factor_tile<N, L, CG, POLICY, ARITH>(a, gl, bc, piv_rec);
+
+  if (mat_ok) {
+    const int64_t o = args.out_idx ? args.out_idx[i] : i;
+    if (args.out.piv) {
+#pragma unroll
+      for (int q = 0; q < PR; ++q) {
+        const int k = gl + q * L;
+        if (k < N)
+          args.out.piv[o * args.piv_stride + k] = (int8_t)(res.info < 0 ? -1 : piv_rec[q]);
+      }
+    }
+    if (gl == 0) write_result(args.out, o, res);
+  }
+}
+
+template <int N, int L, int CG, int POLICY, int ARITH, int WPB, int OCC>
+__global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) {
+  using T = Tile<N, L, CG>;
+  constexpr int G = T::G;
+  constexpr int NP = T::NPAD + 1;
+  __shared__ double2 bcast[WPB * G * NP];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   const int64_t task = (int64_t)blockIdx.x * WPB + wave;
   if (task * G >= args.count) return;  // whole wave idle (uniform branch)
-  {
-    const int64_t i = task * G + g;
-    const bool mat_ok = i < args.count;
-
-    double2 a[R][C];
-    const double2* __restrict__ base = nullptr;
-    if (mat_ok) base = args.offs ? args.A + args.offs[i] : args.A + i * args.stride;
-    const int64_t ld = args.offs ? (int64_t)N : args.ld;
-    const uint8_t* __restrict__ perm = nullptr;
-    if (mat_ok && args.row_perm)
-      perm = args.row_perm + (args.perm_offs ? args.perm_offs[i] : i * N);
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const int row = rb + t * RG;
-      if (args.src == SRC_FQHE && mat_ok && row < N) {
-        // positions: matrix i's z at A + offs[i] (or i*stride), one per row
-        int src = row;
-        if (perm) src = min((int)perm[row], N - 1);
-        const double2 z = base[src];
-        const double g = fqhe_gauss(z);
-        double2 pw = make_double2(1.0, 0.0);
-#pragma unroll
-        for (int s = 0; s < C; ++s) a[t][s] = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-          if (j % CG == cb) a[t][j / CG] = make_double2(pw.x * g, pw.y * g);
-          pw = cmul_ref(pw, z);
-        }
-      } else if (mat_ok && row < N) {
-        int src = row;
-        if (perm) src = min((int)perm[row], N - 1);
-        const double2* __restrict__ rp = base + (int64_t)src * ld;
-#pragma unroll
-        for (int s = 0; s < C; ++s) {
-          const int col = cb + s * CG;
-          a[t][s] = (col < N) ? rp[col] : make_double2(0.0, 0.0);
-        }
-      } else {
-#pragma unroll
-        for (int s = 0; s < C; ++s)
-          a[t][s] = make_double2((cb + s * CG) == row ? 1.0 : 0.0, 0.0);
-      }
-    }
-
-    int piv_rec[PR];
-#pragma unroll
-    for (int q = 0; q < PR; ++q) piv_rec[q] = -1;
-    const GroupResult res =
-        factor_tile<N, L, CG, POLICY, ARITH>(a, gl, bc, piv_rec);
-
-    if (mat_ok) {
-      const int64_t o = args.out_idx ? args.out_idx[i] : i;
-      if (args.out.piv) {
-#pragma unroll
-        for (int q = 0; q < PR; ++q) {
-          const int k = gl + q * L;
-          if (k < N)
-            args.out.piv[o * args.piv_stride + k] =
-                (int8_t)(res.info < 0 ? -1 : piv_rec[q]);
-        }
-      }
-      if (gl == 0) write_result(args.out, o, res);
-    }
-
-  }
+  det_task<N, L, CG, POLICY, ARITH>(args, task, bcast + wave * G * NP, lane);
 }
 
 // Candidate launch shapes per n, as (lanes per matrix L, lane-columns CG,

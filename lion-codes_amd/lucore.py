@@ -486,7 +486,7 @@ def read_matrices(path, count: int, n: int, fmt: str = "complex64", rank: int = 
     return flat[: (vals + 1) // 2]
 
 
-MIXED_MODES = {"graph": 0, "streams": 1, "plain": 2}
+MIXED_MODES = {"graph": 0, "streams": 1, "plain": 2, "persistent": 3}
 SOURCES = {"matrix": 0, "fqhe": 1}
 
 

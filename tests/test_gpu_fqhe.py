@@ -112,7 +112,8 @@ def test_mixed_fqhe_equals_uniform(lc, wl, dev):
     plan = lc.MixedPlan(ns, offs, nstreams=3)
     res = lc.alloc_result(len(ns), 64, dev)
     side = torch.cuda.Stream()
-    for mode in ("plain", "graph"):
+    for mode in ("plain", "graph", "persistent"):
+        res.mant.zero_()
         with torch.cuda.stream(side):
             plan.execute(zd, res, mode=mode, stream=side, source="fqhe")
         side.synchronize()
