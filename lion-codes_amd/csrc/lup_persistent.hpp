@@ -12,10 +12,11 @@
 // the loop when the counter has passed the class's last task.
 //
 // A class kernel carries the register allocation of its largest case, so
-// the classes are cut where the per-size kernels change occupancy: n <= 16,
-// 17..32, 33..48 and 49..57 run two waves per SIMD, 58..64 one.  Under the
-// 256-register cap the class kernels spill more than the per-size kernels
-// of the same n (the task loop holds values across its body).
+// the classes are cut where the per-size kernels change occupancy.  n <= 16
+// and 17..32 run two waves per SIMD.  33..48, 49..57 and 58..64 run one:
+// under the 256-register cap of two waves the class kernels spill far more
+// than the per-size kernels of the same n (528-952 B per lane, the task loop
+// holding values across its body), and measured slower than one wave.
 #pragma once
 
 #include "lup_kernel.hpp"
@@ -25,7 +26,7 @@ namespace lufqhe {
 constexpr int kPersistClasses = 5;
 constexpr int kPersistLo[kPersistClasses] = {1, 17, 33, 49, 58};
 constexpr int kPersistHi[kPersistClasses] = {16, 32, 48, 57, 64};
-constexpr int kPersistOcc[kPersistClasses] = {2, 2, 2, 2, 1};
+constexpr int kPersistOcc[kPersistClasses] = {2, 2, 1, 1, 1};
 
 constexpr int persist_class(int n) {
   int c = 0;
