@@ -194,7 +194,9 @@ int32_t lufqhe_fqhe_build(const void *d_z, int64_t count, int32_t n,
  *                         zeroes.  Captured and replayed like GRAPH (the
  *                         per-bucket arguments are written to device memory,
  *                         synchronously, when the pointers change).  Fused
- *                         arithmetic with either policy.
+ *                         arithmetic with either policy; a plan with 2^31 or
+ *                         more tasks in one class refuses this mode
+ *                         (LUFQHE_ERROR_UNSUPPORTED).
  * A plan is not re-entrant: one execute at a time per plan, on the device
  * that was current when it was created (else LUFQHE_ERROR_INVALID_VALUE). */
 #define LUFQHE_MIXED_GRAPH 0

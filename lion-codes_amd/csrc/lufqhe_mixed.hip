@@ -170,6 +170,13 @@ hipError_t build_persistent(lufqhe_mixed_plan& p) {
       pc.total_tasks += (b.count + g - 1) / g;
       pc.cost += b.cost;
     }
+    // the class counter is 32-bit and every wave adds one past the end: a
+    // class too large for it gets no persistent launch (the mode is refused)
+    if (pc.total_tasks >= (int64_t{1} << 31)) {
+      p.pclasses.clear();
+      p.h_bucket_index.clear();
+      return hipSuccess;
+    }
     const PersistEntry& e = g_persist[p.variant][c];
     if (!e.launch) return hipErrorInvalidDeviceFunction;
     int per_cu = 0;
@@ -378,7 +385,8 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan* plan, const void* d_A, void* d_m
   } else if (mode == LUFQHE_MIXED_GRAPH || mode == LUFQHE_MIXED_PERSISTENT) {
     if (stream == nullptr) return LUFQHE_ERROR_INVALID_VALUE;  // no capture on stream 0
     const bool persistent = mode == LUFQHE_MIXED_PERSISTENT;
-    if (persistent && plan->variant >= kPersistVariants) return LUFQHE_ERROR_UNSUPPORTED;
+    if (persistent && (plan->variant >= kPersistVariants || plan->pclasses.empty()))
+      return LUFQHE_ERROR_UNSUPPORTED;
     if (!plan->exec || plan->g_mode != mode || plan->g_A != d_A || plan->g_src != src ||
         !same_out(plan->g_out, out)) {
       if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
