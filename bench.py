@@ -12,8 +12,11 @@ GPU measurements, and the CPU baseline leg.
 
 The CPU baseline leg runs last and is the only part that uses the oracle:
 it times the restated reference Crout on this host's cores and, as the
-checker, compares samples of every GPU leg's results with the oracle
-(accuracy against the long-double truth, agreement with the Crout).
+checker, compares every result of the n=32, n=16 and mixed-size legs with
+the oracle: the pivot sequence against the restated reference Crout (exact
+ties included; a difference whose first step has criteria within 1e-12 of
+each other is counted as an excluded near-tie) and |det/det_ref - 1|
+against both the Crout and the long-double truth.
 """
 from __future__ import annotations
 
@@ -53,28 +56,45 @@ def det_flops(n: int) -> float:
     return 8.0 * n ** 3 / 3.0 - n * n + 13.0 * n / 3.0 + 6.0 * (n - 1)
 
 
-def host_cores() -> int:
+# The GPU pool gives one GPU's process a share of 16 host cores, whatever
+# nproc shows; the CPU baseline uses that share (and records both counts).
+POOL_CORE_SHARE = 16
+PIVOT_TIE_MARGIN = 1e-12   # relative gap below which a pivot choice is a near-tie
+
+
+def host_cores() -> tuple[int, dict]:
     try:
-        c = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        c = os.cpu_count() or 1
-    return max(1, min(16, c))
+        aff = os.cpu_count() or 1
+    cores = max(1, min(POOL_CORE_SHARE, aff))
+    return cores, {"nproc": os.cpu_count(), "affinity": aff, "pool_share": POOL_CORE_SHARE,
+                   "used": cores}
 
 
 def cpu_model() -> str:
+    """The model string plus family/model/stepping (a virtualised host may
+    report a bare 'AMD EPYC')."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping") and k not in fields:
+                    fields[k] = v
+            if not line.strip() and fields:
+                break
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    return (f"{fields.get('model name', 'unknown')} (family {fields.get('cpu family', '?')}, "
+            f"model {fields.get('model', '?')}, stepping {fields.get('stepping', '?')})")
 
 
-def cpu_baseline(A_host: np.ndarray, cores: int) -> dict:
+def cpu_baseline(A_host: np.ndarray) -> dict:
     """Restated reference CPU Crout (FP64, OpenMP over matrices) on a bounded
     sample of the same batch; median of 5 timed repetitions after a warm-up."""
     from oracle import oracle
+    cores, counts = host_cores()
     oracle.crout_ref(A_host[:256], nthreads=cores)
     ts = []
     for _ in range(5):
@@ -89,33 +109,68 @@ def cpu_baseline(A_host: np.ndarray, cores: int) -> dict:
     return {"value": len(A_host) / t, "unit": "det/s", "cores": cores, "kind": "port",
             "sample": f"first {len(A_host)} matrices of the n=32 batch, median of 5 runs; "
                       f"1 thread: {len(one) / t1:.0f} det/s on 2048",
+            "one_thread_det_per_s": len(one) / t1, "host_cores": counts,
             "cpu_model": cpu_model()}
 
 
-def cpu_checks(out: dict, samples: dict) -> None:
-    """The checker half of the CPU baseline leg: the GPU results sampled by
-    each leg against the oracle, written into that leg's entry of `out`."""
+def err_stats(e: np.ndarray) -> dict:
+    return {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
+            "max": float(e.max()), "count_gt_1e-12": int((e > 1e-12).sum())}
+
+
+def parity_report(A: np.ndarray, la, ph, piv) -> dict:
+    """One uniform group of GPU results against the oracle: pivot parity with
+    the restated reference Crout (near-ties within PIVOT_TIE_MARGIN excluded
+    and counted) and |det/det_ref - 1| against the Crout and the long-double
+    truth, over every matrix."""
     from oracle import oracle
+    ref = oracle.crout_ref_gaps(A)
+    par = oracle.pivot_parity(piv, ref, PIVOT_TIE_MARGIN)
+    cr = oracle.crout_ref(A)
+    la_c, ph_c = oracle.log_form(cr["mant"], cr["exp"])
+    t = oracle.truth(A)
+    return {"pivots": par, "e_crout": log_err_vs(la, ph, la_c, ph_c),
+            "e_truth": log_err_vs(la, ph, t["logabs"], t["phase"])}
+
+
+def summarize(parts: list[dict], masks: list[np.ndarray] | None = None) -> dict:
+    piv = {k: sum(p["pivots"][k] for p in parts)
+           for k in ("matrices", "compared", "differing", "excluded_near_ties", "mismatches",
+                     "matrices_with_exact_ties")}
+    piv["near_tie_margin"] = PIVOT_TIE_MARGIN
+    out = {"pivot_parity_vs_crout_ref": piv}
+    ec = np.concatenate([p["e_crout"] for p in parts])
+    et = np.concatenate([p["e_truth"] for p in parts])
+    if masks is None:
+        out["rel_err_vs_crout_ref"] = err_stats(ec)
+        out["rel_err_vs_long_double"] = err_stats(et)
+    else:
+        well = np.concatenate(masks)
+        out["rel_err_vs_crout_ref"] = err_stats(ec[well])
+        out["rel_err_vs_long_double"] = err_stats(et[well])
+        out["near_singular"] = {"count": int((~well).sum()),
+                                "rel_err_vs_long_double": err_stats(et[~well]),
+                                "rel_err_vs_crout_ref": err_stats(ec[~well])}
+    return out
+
+
+def cpu_checks(out: dict, samples: dict) -> None:
+    """The checker half of the CPU baseline leg: the GPU results of each leg
+    against the oracle, written into that leg's entry of `out`."""
     if "uniform" in samples:
-        A_s, la, ph = samples["uniform"]
-        t = oracle.truth(A_s)
-        e = log_err_vs(la, ph, t["logabs"], t["phase"])
-        out["accuracy"] = {"vs": "long-double LUP truth", "sample": int(len(A_s)),
-                           "median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)),
-                           "max": float(e.max()), "frac_le_1e-12": float(np.mean(e <= 1e-12))}
+        A, la, ph, piv = samples["uniform"]
+        out["accuracy"] = dict(summarize([parity_report(A, la, ph, piv)]),
+                               scope=f"all {len(A)} matrices of the timed batch")
     if "small" in samples:
-        A_s, det = samples["small"]
-        o = oracle.crout_ref(A_s)
-        out["small_batch_n16"]["vs_crout_ref_max_rel"] = float(np.abs(det / o["det"] - 1.0).max())
+        A, la, ph, piv = samples["small"]
+        out["small_batch_n16"]["accuracy"] = dict(summarize([parity_report(A, la, ph, piv)]),
+                                                  scope=f"all {len(A)} matrices")
     if "mixed" in samples:
-        sub, sub_ns, sub_offs, la, ph, well = samples["mixed"]
-        t = oracle.truth_mixed(sub, sub_ns, sub_offs)
-        e = log_err_vs(la, ph, t["logabs"], t["phase"])
-        out["mixed"]["accuracy"] = {
-            "vs": "long-double LUP truth", "sample": len(sub_ns),
-            "median": float(np.median(e[well])), "p99": float(np.quantile(e[well], 0.99)),
-            "frac_le_1e-12": float(np.mean(e[well] <= 1e-12)),
-            "near_singular_median": float(np.median(e[~well])) if (~well).any() else None}
+        groups = samples["mixed"]
+        parts = [parity_report(A, la, ph, piv) for (A, la, ph, piv, _) in groups]
+        out["mixed"]["accuracy"] = dict(
+            summarize(parts, [w for (*_, w) in groups]),
+            scope="all matrices of the mixed batch; near-singular ones (cond ~ 1e8) apart")
 
 
 def time_events(fn, stream, reps: int) -> float:
@@ -165,7 +220,9 @@ def bench_small_batch(lc, wl, dev, reps: int, samples: dict) -> dict:
     s = torch.cuda.current_stream()
     lc.det_batched(A, out=res)
     ms = time_events(lambda: lc.det_batched(A, out=res), s, reps)
-    samples["small"] = (A.cpu().numpy(), res.det().cpu().numpy())
+    rp = lc.det_batched(A, pivots=True)
+    samples["small"] = (A.cpu().numpy(), rp.logabs.cpu().numpy(), rp.phase.cpu().numpy(),
+                        rp.piv.cpu().numpy())
     return {"workload": "4096 complex128 16x16, re,im ~ N(0,1), seed 11, one launch",
             "ms": ms, "det_per_s": 4096 / (ms * 1e-3)}
 
@@ -208,19 +265,27 @@ def bench_mixed(lc, wl, dev, reps: int, samples: dict) -> dict:
         ms = time_events(repacked, side, reps)
         out["repacked_per_size"] = {"ms": ms, "det_per_s": count / (ms * 1e-3)}
     side.synchronize()
+    # every matrix with its pivots (the per-size kernels the plans launch),
+    # for the CPU leg's parity check; the plan's own outputs are compared
+    # with these bit for bit first
+    mant_all = res.mant.cpu().numpy()
+    checks = []
+    for n, el, c in groups:
+        idx = np.nonzero(ns == n)[0]
+        A_n = flat[el].view(c, n, n)
+        rp = lc.det_batched(A_n, pivots=True)
+        same = np.array_equal(rp.mant.cpu().numpy().view(np.uint64),
+                              mant_all[idx].view(np.uint64))
+        checks.append(same)
+        samples.setdefault("mixed", []).append(
+            (A_n.cpu().numpy(), rp.logabs.cpu().numpy(), rp.phase.cpu().numpy(),
+             rp.piv.cpu().numpy(), ~sing[idx]))
+    out["plan_equals_per_size_bitwise"] = bool(all(checks))
+    del groups, outs
     out["speedup_graph_vs_plain"] = out["plain"]["ms"] / out["graph"]["ms"]
     out["speedup_persistent_vs_plain"] = out["plain"]["ms"] / out["persistent"]["ms"]
     out["speedup_graph_vs_repacked"] = out["repacked_per_size"]["ms"] / out["graph"]["ms"]
     assert int((res.info != 0).sum()) == 0
-    # a sample for the CPU leg's accuracy check, by matrix class
-    rng = np.random.default_rng(0)
-    idx = np.sort(rng.choice(count, 1000, replace=False))
-    sub_ns = ns[idx]
-    sub_offs = np.zeros(len(idx), np.int64)
-    sub_offs[1:] = np.cumsum(sub_ns.astype(np.int64) ** 2)[:-1]
-    sub = torch.cat([flat[int(offs[i]):int(offs[i]) + int(ns[i]) ** 2] for i in idx]).cpu().numpy()
-    samples["mixed"] = (sub, sub_ns, sub_offs, res.logabs.cpu().numpy()[idx],
-                        res.phase.cpu().numpy()[idx], ~sing[idx])
     plan.close()
     del flat
     torch.cuda.empty_cache()
@@ -390,12 +455,16 @@ def main() -> None:
                      "bytes_per_det": det_bytes(N), "flops_per_det": det_flops(N)},
     }
     samples = {}
-    acc_idx = torch.arange(0, BATCH, BATCH // 512, device=dev)[:512]
-    samples["uniform"] = (A[acc_idx].cpu().numpy(), res.logabs[acc_idx].cpu().numpy(),
-                          res.phase[acc_idx].cpu().numpy())
+    A_host = None
+    if not args.no_cpu:
+        # every matrix of the timed batch, with its pivots, for the CPU leg
+        rp = lc.det_batched(A, pivots=True)
+        A_host = A.cpu().numpy()
+        samples["uniform"] = (A_host, rp.logabs.cpu().numpy(), rp.phase.cpu().numpy(),
+                              rp.piv.cpu().numpy())
+        del rp
     out["small_batch_n16"] = bench_small_batch(lc, wl, dev, max(5, args.steps), samples)
     out["h2d_inclusive"] = bench_h2d(lc, A, res, 5)
-    A_host = None if args.no_cpu else A[:16384].cpu().numpy()
     legs = [] if args.no_extra else [x for x in args.legs.split(",") if x]
     if legs:
         del A
@@ -412,7 +481,7 @@ def main() -> None:
         out["serving"] = bench_serving(lc, dev)
     if not args.no_cpu:
         print("# CPU baseline and checks", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(A_host, host_cores())
+        out["cpu_baseline"] = cpu_baseline(A_host[:16384])
         cpu_checks(out, samples)
     print(json.dumps(out), flush=True)
 

@@ -6,7 +6,7 @@
 //                               LU-fqhe/src/LUCoreGPU.cu:38-106
 //   LUCoreGPU::globalReduction  LU-fqhe/inc/LUCoreGPU.h:50,
 //                               LU-fqhe/src/LUCoreGPU.cu:113-156
-//   RunInfo                     LU-fqhe/inc/RunInfo.h, LU-fqhe/src/RunInfo.cpp:46-129
+//   RunInfo                     run_info.hpp
 //
 // Semantics follow the reference build (WRITE_DET, PERFORM_PROD,
 // LU-fqhe/Makefile:3): CALL leaves det(M_m) in M_m[0][0] and, for even m,
@@ -21,28 +21,7 @@
 #include <string>
 
 #include "lufqhe.h"
-
-// Sizes of one run: the part of the reference's RunInfo the GPU path reads.
-// The reference derives loadSize from the __PADDED__ build macro
-// (LU-fqhe/src/RunInfo.cpp:72-76); here it is a constructor flag.
-class RunInfo {
- public:
-  RunInfo(int nMperBl, int matSide, int nElem, bool padded = false, int nodes = 1);
-
-  int getNMperBl() const { return nMperBl_; }
-  int getMatSide() const { return matSide_; }
-  int getLoadSize() const { return loadSize_; }
-  int getNElem() const { return nElem_; }
-  int getNElemPnode() const { return nElemPNode_; }
-  int getScrSpace() const { return scrSpace_; }
-  int getScrSpace2() const { return scrSpace2_; }
-
-  // Smallest power of two >= n (LU-fqhe/src/RunInfo.cpp:119-129).
-  static int p2Above(int n);
-
- private:
-  int nMperBl_, matSide_, nElem_, nElemPNode_, scrSpace_, scrSpace2_, loadSize_;
-};
+#include "run_info.hpp"
 
 namespace LUCoreGPU {
 

@@ -15,11 +15,14 @@
  *   exp    int32
  *   logabs float64 log|det| (-inf for det == 0)
  *   phase  float64 arg det in (-pi, pi]
- *   info   int32: 0 ok; k+1 the pivot of step k had |p|^2 == 0 (det = 0; the
- *          reference divides by zero there, LU-fqhe/src/LUCoreCPU.cu:243);
- *          -(i+1) pivot policy CROUT_REF only: row i has max_j |Re a_ij| <
- *          1e-24, which the reference answers with exit(0)
- *          (LU-fqhe/src/LUCoreCPU.cu:158); outputs are NaN
+ *   info   int32: 0 ok; k+1 the pivot of step k was zero (det = 0; the
+ *          reference divides by zero there, LU-fqhe/src/LUCoreCPU.cu:243 --
+ *          in REF arithmetic, as there, a pivot whose |p|^2 underflows to 0
+ *          counts as zero); -(i+1) pivot policy CROUT_REF only: row i has
+ *          max_j |Re a_ij| < 1e-24, which the reference answers with exit(0)
+ *          (LU-fqhe/src/LUCoreCPU.cu:158); LUFQHE_INFO_BAD_PERM: a row
+ *          permutation entry of this matrix is >= n.  For negative codes the
+ *          determinant outputs are NaN and the pivots -1
  *   piv    int8, n per matrix: the row position exchanged with position k at
  *          step k (the reference's imax, LU-fqhe/src/LUCoreCPU.cu:209-216)
  *
@@ -46,17 +49,22 @@ extern "C" {
 #define LUFQHE_ERROR_IO 6
 
 /* Pivot policies.  CROUT_REF is the reference's rule: argmax over remaining
- * rows of |Re u_ik| / s_i, s_i = max_j |Re a_ij| of the original row, ties to
- * the last position (LU-fqhe/src/LUCoreCPU.cu:145-213).  ABS: argmax |u_ik|^2,
- * ties to the last position. */
+ * rows of the correctly rounded |Re u_ik| / s_i, s_i = max_j |Re a_ij| of the
+ * original row, exact ties to the last position (LU-fqhe/src/LUCoreCPU.cu:
+ * 145-213), in both arithmetic modes.  ABS: LAPACK zgetf2's rule, argmax of
+ * |Re u_ik| + |Im u_ik| (izamax's cabs1), exact ties to the first position. */
 #define LUFQHE_POLICY_CROUT_REF 0
 #define LUFQHE_POLICY_ABS 1
 /* Arithmetic.  FUSED: FP64 with fused multiply-adds (4 FMAs per complex
- * update, one reciprocal of the pivot per step).  REF: the reference's own
- * formulas and rounding steps (LU-fqhe/src/LUCoreCPU.cu:173-176,243-246,
- * 258-259); bit-identical to an FP64 build of LUCoreCPU::determinant. */
+ * update, one reciprocal of the pivot per step, the pivot scaled by a power
+ * of two before |p|^2 so that it neither overflows nor underflows).  REF: the
+ * reference's own formulas and rounding steps (LU-fqhe/src/LUCoreCPU.cu:
+ * 173-176,243-246,258-259); bit-identical to an FP64 build of
+ * LUCoreCPU::determinant. */
 #define LUFQHE_ARITH_FUSED 0
 #define LUFQHE_ARITH_REF 1
+/* info code of a matrix whose row-permutation entry is out of range. */
+#define LUFQHE_INFO_BAD_PERM (-1000)
 
 /* Library version (major*10000 + minor*100 + patch). */
 int32_t lufqhe_version(void);
@@ -134,6 +142,33 @@ int32_t lufqhe_pair_sum(const void *d_mant, const int32_t *d_exp,
                         int64_t count, void *d_work, void *d_out_mant,
                         int32_t *d_out_exp, hipStream_t stream);
 
+/* ---- the reference's CPU Crout, on the host -----------------------------
+ * LUCoreCPU::determinant (LU-fqhe/src/LUCoreCPU.cu:112-267) in FP64: left-
+ * looking Crout, rows scaled by their largest |Re|, the `>=` pivot scan,
+ * full row swaps, complex-divide column scaling, sequential diagonal
+ * product, every operation rounded as the reference writes it; the results
+ * equal the GPU's REF arithmetic bit for bit.  Host pointers; the input is
+ * read only; nthreads <= 0 uses every hardware thread.
+ *
+ * lufqhe_cpu_crout_det: per matrix (layout as lufqhe_det_batched) the plain
+ * product h_det (complex128, may overflow), h_mant * 2^h_exp, h_info (as the
+ * GPU's, without the permutation code) and h_piv (n int8); any output may
+ * be NULL.  Replaces the loop of LUCoreCPU::CALL (LU-fqhe/src/LUCoreCPU.cu:
+ * 69-93) with per-matrix outputs.
+ * lufqhe_cpu_pair_sum: *h_out = sum over i < num/2 of det(M_2i) *
+ * det(M_2i+1), the value LUCoreCPU::CALL returns (LU-fqhe/inc/LUCoreCPU.h:
+ * 26, LU-fqhe/src/LUCoreCPU.cu:39-107), with the pair products added in
+ * matrix order (deterministic for any thread count); *h_first_bad (may be
+ * NULL) gets the first pair holding a row the reference would exit(0) on,
+ * or -1. */
+int32_t lufqhe_cpu_crout_det(const void *h_A, int64_t count, int32_t n,
+                             int64_t ld, int64_t stride, void *h_det,
+                             void *h_mant, int32_t *h_exp, int32_t *h_info,
+                             int8_t *h_piv, int32_t nthreads);
+int32_t lufqhe_cpu_pair_sum(const void *h_A, int64_t num, int32_t n,
+                            int64_t ld, int64_t stride, void *h_out,
+                            int32_t *h_first_bad, int32_t nthreads);
+
 /* ---- FQHE matrices built on the device ---------------------------------
  * LU-fqhe's matrices are Laughlin-type wavefunction matrices
  * A_ij = z_i^j exp(-|z_i|^2/4), j = 0..n-1, of particle positions z_i
@@ -145,8 +180,9 @@ int32_t lufqhe_pair_sum(const void *d_mant, const int32_t *d_exp,
  *
  * Matrix i's positions start at d_z + i*z_stride (complex elements,
  * z_stride >= n or 0 to share one configuration); with d_row_perm (n bytes
- * per matrix, may be NULL) row r of matrix i is particle d_row_perm[i*n + r]
- * (values above n-1 read as n-1).  Powers are sequential products
+ * per matrix, may be NULL) row r of matrix i is particle d_row_perm[i*n + r];
+ * an entry >= n makes that matrix's info LUFQHE_INFO_BAD_PERM (NaN outputs),
+ * and lufqhe_fqhe_build writes that row as NaN.  Powers are sequential products
  * z^j = z^(j-1) * z, so entries do not depend on the launch shape.
  *
  * lufqhe_fqhe_det: determinants as lufqhe_det_batched.

@@ -11,6 +11,8 @@ from .lucore import (  # noqa: F401
     MixedPlan,
     Server,
     alloc_result,
+    cpu_crout_det,
+    cpu_pair_sum,
     det_batched,
     apply_shape_table,
     det_inplace,

@@ -130,6 +130,20 @@ def build_driver(force: bool = True, verbose: bool = False) -> Path:
     return DRIVER
 
 
+CPU_DRIVER_SRC = ROOT / "tests" / "cpp" / "lucore_cpu_driver.cpp"
+CPU_DRIVER = ROOT / "tests" / "cpp" / "lucore_cpu_driver"
+
+
+def build_cpu_driver(force: bool = True, verbose: bool = False) -> Path:
+    """C++ test driver of include/lucore_cpu.hpp, linked against the library."""
+    deps = [CPU_DRIVER_SRC, LIB, ROOT / "include" / "lucore_cpu.hpp", ROOT / "include" / "lufqhe.h"]
+    if force or _stale(CPU_DRIVER, deps):
+        _run([hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(CPU_DRIVER_SRC), "-o",
+              str(CPU_DRIVER), f"-L{PKG}", "-llufqhe", "-Wl,-rpath,$ORIGIN/../../lion-codes_amd"],
+             verbose)
+    return CPU_DRIVER
+
+
 def build_oracle(force: bool = True, verbose: bool = False) -> Path:
     src = ORACLE_DIR / "lufqhe_oracle.c"
     deps = [src, ORACLE_DIR / "lufqhe_oracle.h"]
@@ -145,6 +159,7 @@ def build(force: bool = True, jobs: int | None = None, verbose: bool = False) ->
     build_oracle(force=force, verbose=verbose)
     build_library(force=force, jobs=jobs, verbose=verbose)
     build_driver(force=force, verbose=verbose)
+    build_cpu_driver(force=force, verbose=verbose)
 
 
 if __name__ == "__main__":

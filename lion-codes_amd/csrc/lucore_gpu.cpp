@@ -4,22 +4,6 @@
 #include <mutex>
 #include <string>
 
-RunInfo::RunInfo(int nMperBl, int matSide, int nElem, bool padded, int nodes)
-    : nMperBl_(nMperBl),
-      matSide_(matSide),
-      nElem_(nElem),
-      nElemPNode_(nodes > 0 ? nElem / nodes : nElem),
-      scrSpace_(p2Above(matSide)),
-      scrSpace2_(16),
-      loadSize_(padded ? p2Above(matSide) : matSide) {}
-
-int RunInfo::p2Above(int n) {
-  if (n < 0) return 0;
-  int p = 1;
-  while (p < n) p <<= 1;
-  return p;
-}
-
 namespace LUCoreGPU {
 namespace {
 thread_local int t_policy = LUFQHE_POLICY_CROUT_REF;

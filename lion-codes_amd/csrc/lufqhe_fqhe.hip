@@ -14,8 +14,9 @@
 namespace lufqhe {
 
 // Matrix i's row r comes from position z[i*z_stride + p], p = row_perm[i*n+r]
-// (or r), clamped to n-1.  Output matrix i at out + i*n*n, ld = n.  One thread
-// per row.
+// (or r); a permutation entry >= n gives a row of NaN (the determinant
+// kernels report such a matrix with info kInfoBadPerm).  Output matrix i at
+// out + i*n*n, ld = n.  One thread per row.
 __global__ void __launch_bounds__(256) fqhe_build_kernel(const double2* __restrict__ z,
                                                          int64_t count, int n, int64_t z_stride,
                                                          const uint8_t* __restrict__ row_perm,
@@ -25,11 +26,15 @@ __global__ void __launch_bounds__(256) fqhe_build_kernel(const double2* __restri
   const int64_t i = r / n;
   const int row = (int)(r - i * n);
   int src = row;
-  if (row_perm) src = min((int)row_perm[r], n - 1);
+  if (row_perm) src = row_perm[r];
+  double2* o = out + r * n;
+  if (src >= n) {
+    for (int j = 0; j < n; ++j) o[j] = make_double2(__builtin_nan(""), __builtin_nan(""));
+    return;
+  }
   const double2 zi = z[i * z_stride + src];
   const double g = fqhe_gauss(zi);
   double2 pw = make_double2(1.0, 0.0);
-  double2* o = out + r * n;
   for (int j = 0; j < n; ++j) {
     o[j] = make_double2(pw.x * g, pw.y * g);
     pw = cmul_ref(pw, zi);

@@ -45,6 +45,14 @@ enum Arith { ARITH_FUSED = 0, ARITH_REF = 1 };
 constexpr double kDblMax = 1.7976931348623157e308;
 constexpr double kRowScaleMin = 1e-24;  // LU-fqhe/src/LUCoreCPU.cu:158
 constexpr double kLn2 = 0.69314718055994530942;
+// Range in which the corrected product x * RN(1/s) of quot_rn below is the
+// correctly rounded x / s: the reciprocal, the first product and the
+// residual stay normal.  Outside it (and for x == 0) the quotient is an
+// IEEE division.
+constexpr double kQuotLo = 0x1p-880;
+constexpr double kQuotHi = 0x1p960;
+// info code of a matrix whose row-permutation table has an entry >= n.
+constexpr int kInfoBadPerm = -1000;
 
 // Output pointers of one launch.  Any of them may be null except mant.
 struct DetOut {
@@ -52,7 +60,8 @@ struct DetOut {
   int32_t* expo;
   double* logabs;    // log|det|
   double* phase;     // arg det in (-pi, pi]
-  int32_t* info;     // 0, k+1 (zero pivot at step k), -(i+1) (row i rejected)
+  int32_t* info;     // 0, k+1 (zero pivot at step k), -(i+1) (row i rejected),
+                     // kInfoBadPerm (row-permutation entry out of range)
   int8_t* piv;       // n bytes per matrix: reference position swapped at step k
 };
 
@@ -84,6 +93,13 @@ constexpr int kMirror = 0x140;
 __device__ __forceinline__ double vmax_f64(double a, double b) {
   double r;
   asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// max(|a|, |b|) in one instruction (abs source modifiers).
+__device__ __forceinline__ double vmax_abs_f64(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, |%1|, |%2|" : "=v"(r) : "v"(a), "v"(b));
   return r;
 }
 
@@ -146,9 +162,29 @@ __device__ __forceinline__ double2 sel2(bool c, double2 x, double2 y) {
   return make_double2(c ? x.x : y.x, c ? x.y : y.y);
 }
 
+// Correctly rounded x / s for x >= 0 from y = RN(1/s): q0 = RN(x*y) is within
+// one ulp of RN(x/s), and one residual correction, r = x - s*q0 (exact in
+// an FMA) and q = RN(q0 + r*y), gives RN(x/s) (Markstein's correction,
+// P. Markstein, IBM J. Res. Dev. 34(1), 1990).  Valid while y, q0 and the
+// residual stay normal: y in [2^-960, 2^960] (the caller passes y = NaN for
+// a row scale outside it) and q0 in [kQuotLo, kQuotHi], which bounds x from
+// below through y <= 2^80 (s >= 1e-24).  Anything else -- x == 0, tiny or
+// huge quotients, inf, NaN -- takes an IEEE division, a branch no lane of an
+// ordinary batch enters.  The result equals the reference's `fabs(x)/s`
+// (LU-fqhe/src/LUCoreCPU.cu:205) bit for bit, so exact criterion ties break
+// as the reference's `>=` scan does.
+__device__ __forceinline__ double quot_rn(double x, double s, double y, bool need) {
+  const double q0 = x * y;
+  const double r = fma(-q0, s, x);
+  double q = fma(r, y, q0);
+  const bool ok = (q0 >= kQuotLo) & (q0 <= kQuotHi);
+  if (__builtin_expect(need && !ok, 0)) q = x / s;
+  return q;
+}
+
 // mant * 2^e renormalisation, exact (power-of-two scaling).
 __device__ __forceinline__ void renorm(double& x, double& y, int& e) {
-  const double m = fmax(fabs(x), fabs(y));
+  const double m = vmax_abs_f64(x, y);           // a NaN operand yields the other
   const int k = __builtin_amdgcn_frexp_exp(m);  // 0 for 0, inf, nan
   x = ldexp(x, -k);
   y = ldexp(y, -k);
@@ -274,6 +310,15 @@ __device__ __forceinline__ int max_level_i(int i, int v) {
   }
 }
 
+// Tie key of a candidate position: CROUT_REF keeps the last position among
+// exact ties (the reference's `>=` scan, LU-fqhe/src/LUCoreCPU.cu:207-212),
+// ABS the first (LAPACK izamax's strict `>`); the search takes the largest
+// key.
+template <int POLICY>
+__device__ __forceinline__ int tie_key(int pos) {
+  return POLICY == POLICY_ABS ? 255 - pos : pos;
+}
+
 template <int L, int CG, int POLICY, int ARITH, int R, int C>
 __device__ __forceinline__ void search_stage(int j, SearchState& st, const double2 (&a)[R][C],
                                              const int (&pos)[R], const bool (&act)[R],
@@ -287,27 +332,28 @@ __device__ __forceinline__ void search_stage(int j, SearchState& st, const doubl
 #pragma unroll
     for (int t = 0; t < R; ++t) {
       const double2 v = a[t][sk];
+      const bool cand = act[t] && cb == ck;
       double c;
       if constexpr (POLICY == POLICY_CROUT_REF) {
         if constexpr (ARITH == ARITH_FUSED)
-          c = fabs(v.x) * rs[t];
+          c = quot_rn(fabs(v.x), sc[t], rs[t], cand);
         else
           c = fabs(v.x) / sc[t];
       } else {
-        if constexpr (ARITH == ARITH_FUSED)
-          c = fma(v.x, v.x, v.y * v.y);
-        else
-          c = v.x * v.x + v.y * v.y;
+        // |Re| + |Im| (LAPACK's cabs1): no overflow below DBL_MAX / 2 and
+        // invariant under power-of-two scaling
+        c = fabs(v.x) + fabs(v.y);
       }
       c = fmin(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
-      c = (act[t] && cb == ck) ? c : -1.0;
+      c = cand ? c : -1.0;
+      const int key = tie_key<POLICY>(pos[t]);
       if (t == 0) {
         cbest = c;
-        pbest = pos[0];
+        pbest = key;
       } else {
-        const bool better = (c > cbest) || (c == cbest && pos[t] > pbest);
+        const bool better = (c > cbest) || (c == cbest && key > pbest);
         cbest = better ? c : cbest;
-        pbest = better ? pos[t] : pbest;
+        pbest = better ? key : pbest;
       }
     }
     st.cl = cbest;
@@ -382,7 +428,7 @@ __device__ __forceinline__ int update_and_search(
   if (nch == 0) {
 #pragma unroll
     for (int j = 0; j < NS; ++j) search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, k + 1);
-    return st.pm;
+    return tie_key<POLICY>(st.pm);
   }
   double2 ub[2][CH];
 #pragma unroll
@@ -407,7 +453,7 @@ __device__ __forceinline__ int update_and_search(
       if (c0 + c < C) pin_slot(a, c0 + c);
     pin(st);
   }
-  return st.pm;
+  return tie_key<POLICY>(st.pm);
 }
 
 // Factorisation of one matrix held by a group as a[R][C] (see Tile).  Rows
@@ -450,7 +496,9 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
       const bool bad = act[t] && m < kRowScaleMin;
       if (bad) worst = max(worst, 255 - pos[t]);
       sc[t] = (act[t] && !bad) ? m : 1.0;
-      rs[t] = 1.0 / sc[t];
+      // RN(1/s) for quot_rn; NaN sends a row scale above 2^960 (whose
+      // reciprocal would not be a normal number) to the division
+      rs[t] = sc[t] <= 0x1p960 ? 1.0 / sc[t] : __builtin_nan("");
     }
     const int first_bad = group_max_i<L>(worst);
     if (first_bad >= 0) rej = 255 - first_bad;
@@ -464,7 +512,7 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
 #pragma unroll
     for (int j = 0; j < search_stages<L>(); ++j)
       search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, 0);
-    ppos = st.pm;
+    ppos = tie_key<POLICY>(st.pm);
   }
 
 #pragma unroll
@@ -526,11 +574,21 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
       renorm(dx, dy, de);
     }
 
-    double den;
-    if constexpr (ARITH == ARITH_FUSED)
-      den = fma(p.x, p.x, p.y * p.y);
-    else
+    // |p|^2.  Fused arithmetic first scales p by 2^-kp, kp the exponent of
+    // max(|Re p|, |Im p|), so that |p|^2 neither overflows nor underflows:
+    // it is 0 exactly when p == 0.  The scaling is exact and the reciprocal
+    // is scaled back below, so in the normal range the multipliers have the
+    // bits of the unscaled formula.
+    double den, psx = p.x, psy = p.y;
+    int kp = 0;
+    if constexpr (ARITH == ARITH_FUSED) {
+      kp = __builtin_amdgcn_frexp_exp(vmax_abs_f64(p.x, p.y));  // 0 for 0, inf, nan
+      psx = ldexp(p.x, -kp);
+      psy = ldexp(p.y, -kp);
+      den = fma(psx, psx, psy * psy);
+    } else {
       den = p.x * p.x + p.y * p.y;
+    }
     const bool zero = (den == 0.0);
     if (zero && info == 0) info = k + 1;
 
@@ -540,11 +598,11 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
     if (k + 1 < N) {
       double ir = 0.0, ii = 0.0;
       if constexpr (ARITH == ARITH_FUSED) {
-        // a zero pivot (|p|^2 == 0, also for a nonzero p whose square
-        // underflows) gets exactly zero multipliers: the step changes no row
+        // 1/p = conj(p_s) / |p_s|^2 * 2^-kp; a zero pivot gets exactly zero
+        // multipliers: the step changes no row
         const double rden = zero ? 0.0 : 1.0 / den;
-        ir = p.x * rden;
-        ii = -(p.y * rden);
+        ir = ldexp(psx * rden, -kp);
+        ii = ldexp(-(psy * rden), -kp);
       }
 #pragma unroll
       for (int t = 0; t < R; ++t) {
@@ -678,6 +736,12 @@ __device__ __forceinline__ void det_task(const BatchArgs& args, int64_t task,
   const uint8_t* __restrict__ perm = nullptr;
   if (mat_ok && args.row_perm)
     perm = args.row_perm + (args.perm_offs ? args.perm_offs[i] : i * N);
+  int bad_perm = 0;  // a permutation entry >= N: the matrix gets kInfoBadPerm
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int row = rb + t * RG;
+    if (perm && row < N && perm[row] >= N) bad_perm = 1;
+  }
 #pragma unroll
   for (int t = 0; t < R; ++t) {
     const int row = rb + t * RG;
@@ -714,7 +778,8 @@ __device__ __forceinline__ void det_task(const BatchArgs& args, int64_t task,
   int piv_rec[PR];
 #pragma unroll
   for (int q = 0; q < PR; ++q) piv_rec[q] = -1;
-  const GroupResult res = factor_tile<N, L, CG, POLICY, ARITH>(a, gl, bc, piv_rec);
+  GroupResult res = factor_tile<N, L, CG, POLICY, ARITH>(a, gl, bc, piv_rec);
+  if (args.row_perm && group_max_i<L>(bad_perm) > 0) res.info = kInfoBadPerm;
 
   if (mat_ok) {
     const int64_t o = args.out_idx ? args.out_idx[i] : i;

@@ -117,6 +117,11 @@ def load() -> ctypes.CDLL:
                                          ctypes.POINTER(_i64), _vp]
     lib.lufqhe_load_shape_table.restype = _i32
     lib.lufqhe_load_shape_table.argtypes = [ctypes.c_char_p]
+    lib.lufqhe_cpu_crout_det.restype = _i32
+    lib.lufqhe_cpu_crout_det.argtypes = [_vp, _i64, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp,
+                                         _i32]
+    lib.lufqhe_cpu_pair_sum.restype = _i32
+    lib.lufqhe_cpu_pair_sum.argtypes = [_vp, _i64, _i32, _i64, _i64, _vp, _vp, _i32]
     lib.lufqhe_fqhe_build.restype = _i32
     lib.lufqhe_fqhe_build.argtypes = [_vp, _i64, _i32, _i64, _vp, _vp, _vp]
     _lib = lib
@@ -357,6 +362,47 @@ def fqhe_build(z: torch.Tensor, row_perm: torch.Tensor | None = None, count: int
     if st != SUCCESS:
         raise LufqheError("lufqhe_fqhe_build", st)
     return A
+
+
+def cpu_crout_det(A, nthreads: int = 0, pivots: bool = False) -> dict:
+    """The reference CPU Crout (LUCoreCPU::determinant) on the host through
+    the C ABI: ``A`` a numpy complex128 array [B, n, n] (any row stride with
+    unit column stride).  Returns dict(det, mant, exp, info, piv)."""
+    import numpy as np
+    A = np.asarray(A)
+    if A.dtype != np.complex128 or A.ndim != 3 or A.strides[2] != 16:
+        raise ValueError("A must be a complex128 numpy array [B, n, n] with unit column stride")
+    B, n, n2 = A.shape
+    if n != n2 or A.strides[1] % 16 or A.strides[0] % 16:
+        raise ValueError("A must hold square matrices on complex128 strides")
+    det = np.empty(B, np.complex128)
+    mant = np.empty(B, np.complex128)
+    exp = np.empty(B, np.int32)
+    info = np.empty(B, np.int32)
+    piv = np.empty((B, n), np.int8) if pivots else None
+    st = load().lufqhe_cpu_crout_det(A.ctypes.data, B, n, A.strides[1] // 16, A.strides[0] // 16,
+                                     det.ctypes.data, mant.ctypes.data, exp.ctypes.data,
+                                     info.ctypes.data, None if piv is None else piv.ctypes.data,
+                                     nthreads)
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_cpu_crout_det", st)
+    return dict(det=det, mant=mant, exp=exp, info=info, piv=piv)
+
+
+def cpu_pair_sum(A, nthreads: int = 0) -> tuple[complex, int]:
+    """LUCoreCPU::CALL's value on the host: sum_i det(A[2i]) * det(A[2i+1])
+    (pair products added in matrix order) and the first pair holding a
+    rejected row (-1 if none)."""
+    import numpy as np
+    A = np.ascontiguousarray(A, np.complex128)
+    B, n, _ = A.shape
+    out = np.empty(1, np.complex128)
+    bad = _i32()
+    st = load().lufqhe_cpu_pair_sum(A.ctypes.data, B, n, n, n * n, out.ctypes.data,
+                                    ctypes.byref(bad), nthreads)
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_cpu_pair_sum", st)
+    return complex(out[0]), bad.value
 
 
 class ServeConfig(ctypes.Structure):

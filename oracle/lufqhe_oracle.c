@@ -109,7 +109,7 @@ static void renorm(double *x, double *y, int32_t *e) {
             IM(i, j) = sy;                                                    \
         }                                                                     \
         /* L part and pivot search, LUCoreCPU.cu:188-213 */                   \
-        T big = 0;                                                            \
+        T big = 0, second = -1;                                               \
         for (int i = j; i < n; ++i) {                                         \
             T sx = RE(i, j), sy = IM(i, j);                                   \
             for (int k = 0; k < j; ++k) {                                     \
@@ -122,10 +122,16 @@ static void renorm(double *x, double *y, int32_t *e) {
             IM(i, j) = sy;                                                    \
             T dum = FABS(sx) / holder[i];                                     \
             if (dum >= big) {                                                 \
+                second = (i > j) ? big : second;                              \
                 big = dum;                                                    \
                 imax = i;                                                     \
+            } else if (dum > second) {                                        \
+                second = dum;                                                 \
             }                                                                 \
         }                                                                     \
+        /* relative margin of the choice over the best other candidate */    \
+        if (gap) gap[j] = (j == n - 1) ? INFINITY                             \
+                          : (big > 0 ? (double)((big - second) / big) : 0.0); \
         /* row swap, LUCoreCPU.cu:215-232 */                                  \
         if (j != imax) {                                                      \
             for (int k = 0; k < n; ++k) {                                     \
@@ -162,7 +168,8 @@ static void renorm(double *x, double *y, int32_t *e) {
 #define IM(i, j) a[2 * ((i) * ld + (j)) + 1]
 
 static int crout_ref_z_inplace(double *a, int n, int ld, double *det_plain,
-                               double *mant, int32_t *exp2, int32_t *piv) {
+                               double *mant, int32_t *exp2, int32_t *piv,
+                               double *gap) {
     CROUT_BODY(double, fabs)
     /* determinant, LUCoreCPU.cu:256-265 (plain), and the same sequence
      * carried as mant * 2^e */
@@ -194,6 +201,7 @@ static int crout_ref_z_inplace(double *a, int n, int ld, double *det_plain,
 
 static int crout_ref_c_inplace(float *a, int n, int ld, float *det_plain,
                                int32_t *piv) {
+    double *gap = NULL;
     CROUT_BODY(float, fabsf)
     float dx = 1.0f, dy = 0.0f;
     for (int i = 0; i < n; ++i) {
@@ -219,15 +227,21 @@ static void nan_outputs(double *det_plain, double *mant, int32_t *exp2,
         for (int k = 0; k < n; ++k) piv[k] = -1;
 }
 
-int oracle_crout_ref_z(const double *A, int n, int ld, double *det_plain,
-                       double *mant, int32_t *exp2, int32_t *piv) {
+static int crout_ref_z_gap(const double *A, int n, int ld, double *det_plain,
+                           double *mant, int32_t *exp2, int32_t *piv,
+                           double *gap) {
     if (n < 1 || n > MAXN || ld < n) return -1000;
     double a[2 * MAXN * MAXN];
     for (int i = 0; i < n; ++i)
         memcpy(&a[2 * i * n], &A[2 * (size_t)i * ld], sizeof(double) * 2 * n);
-    int info = crout_ref_z_inplace(a, n, n, det_plain, mant, exp2, piv);
+    int info = crout_ref_z_inplace(a, n, n, det_plain, mant, exp2, piv, gap);
     if (info < 0) nan_outputs(det_plain, mant, exp2, piv, n);
     return info;
+}
+
+int oracle_crout_ref_z(const double *A, int n, int ld, double *det_plain,
+                       double *mant, int32_t *exp2, int32_t *piv) {
+    return crout_ref_z_gap(A, n, ld, det_plain, mant, exp2, piv, NULL);
 }
 
 int oracle_crout_ref_c(const float *A, int n, int ld, float *det_plain,
@@ -249,6 +263,32 @@ int oracle_crout_ref_c(const float *A, int n, int ld, float *det_plain,
 /* 2. Mirror of the HIP kernel's operation order                            */
 /* ------------------------------------------------------------------------ */
 
+/* Exponent k of m with m = f * 2^k, f in [0.5, 1); 0 for 0, inf and NaN (the
+ * convention of the kernel's frexp-exponent instruction). */
+static int frexp_exp0(double m) {
+    int k = 0;
+    if (m != 0.0 && isfinite(m)) (void)frexp(m, &k);
+    return k;
+}
+
+/* The kernel's CROUT_REF criterion in fused arithmetic: x / s from
+ * y = RN(1/s) by one residual correction (Markstein), with an IEEE division
+ * outside the range where that is exact.  Both branches give RN(x / s), the
+ * reference's `fabs(sum.x) / holder[i]` (LU-fqhe/src/LUCoreCPU.cu:205);
+ * tests/test_oracle.py checks the corrected product against the division
+ * on constructed hardest-to-round quotients. */
+double oracle_quot_rn(double x, double s, double y) {
+    const double q0 = x * y;
+    const double r = fma(-q0, s, x);
+    double q = fma(r, y, q0);
+    if (!(q0 >= 0x1p-880 && q0 <= 0x1p960)) q = x / s;
+    return q;
+}
+
+/* RN(1/s) as the kernel keeps it per row: NaN for s > 2^960 sends every
+ * quotient of that row to the division. */
+static double row_recip(double s) { return s <= 0x1p960 ? 1.0 / s : NAN; }
+
 int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
                         double *mant, int32_t *exp2, int32_t *piv) {
     if (n < 1 || n > MAXN || ld < n) return -1000;
@@ -268,7 +308,7 @@ int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
             for (int j = 1; j < n; ++j) big = fmax(big, fabs(a[i][j][0]));
             if (big < 1e-24 && bad < 0) bad = i;
             s[i] = big;
-            rs[i] = 1.0 / big;
+            rs[i] = row_recip(big);
         }
         if (bad >= 0) {
             nan_outputs(NULL, mant, exp2, piv, n);
@@ -283,20 +323,24 @@ int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
     int32_t de = 0;
     int info = 0;
     for (int k = 0; k < n; ++k) {
-        /* pivot search: max criterion, ties to the larger position */
+        /* pivot search: max criterion; exact ties go to the larger position
+         * under CROUT_REF (the reference's `>=`), to the smaller under ABS
+         * (LAPACK izamax) */
         double cmax = -INFINITY;
         int ppos = -1, pr = -1;
         for (int i = 0; i < n; ++i) {
             if (!act[i]) continue;
             double x = a[i][k][0], y = a[i][k][1], c;
             if (policy == ORACLE_POLICY_CROUT_REF)
-                c = (arith == ORACLE_ARITH_FUSED) ? fabs(x) * rs[i]
-                                                  : fabs(x) / s[i];
+                c = (arith == ORACLE_ARITH_FUSED)
+                        ? oracle_quot_rn(fabs(x), s[i], rs[i])
+                        : fabs(x) / s[i];
             else
-                c = (arith == ORACLE_ARITH_FUSED) ? fma(x, x, y * y)
-                                                  : x * x + y * y;
+                c = fabs(x) + fabs(y); /* LAPACK's cabs1 */
             c = fmin(c, DBL_MAX);
-            if (c > cmax || (c == cmax && pos[i] > ppos)) {
+            const int later = (policy == ORACLE_POLICY_ABS) ? pos[i] < ppos
+                                                            : pos[i] > ppos;
+            if (c > cmax || (c == cmax && later)) {
                 cmax = c;
                 ppos = pos[i];
                 pr = i;
@@ -319,17 +363,28 @@ int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
         dx = nx;
         dy = ny;
         renorm(&dx, &dy, &de);
-        /* multipliers and rank-1 trailing update */
-        double den = (arith == ORACLE_ARITH_FUSED) ? fma(px, px, py * py)
-                                                   : px * px + py * py;
+        /* multipliers and rank-1 trailing update; fused arithmetic scales p
+         * by 2^-kp before squaring (kp the exponent of max(|Re p|, |Im p|)),
+         * so |p|^2 is 0 only for p == 0, and scales the reciprocal back */
+        double den;
+        int kp = 0;
+        double psx = px, psy = py;
+        if (arith == ORACLE_ARITH_FUSED) {
+            kp = frexp_exp0(fmax(fabs(px), fabs(py)));
+            psx = ldexp(px, -kp);
+            psy = ldexp(py, -kp);
+            den = fma(psx, psx, psy * psy);
+        } else {
+            den = px * px + py * py;
+        }
         if (den == 0.0) {
             if (!info) info = k + 1;
         } else {
             double ir = 0.0, ii = 0.0;
             if (arith == ORACLE_ARITH_FUSED) {
                 double rden = 1.0 / den;
-                ir = px * rden;
-                ii = -(py * rden);
+                ir = ldexp(psx * rden, -kp);
+                ii = ldexp(-(psy * rden), -kp);
             }
             for (int i = 0; i < n; ++i) {
                 if (!act[i] || i == pr) continue;
@@ -458,6 +513,18 @@ void oracle_crout_ref_batch_z(const double *A, int64_t count, int n, int ld,
                                    mant ? mant + 2 * m : NULL,
                                    exp2 ? exp2 + m : NULL,
                                    piv ? piv + m * n : NULL);
+        if (info) info[m] = r;
+    }
+}
+
+void oracle_crout_ref_gap_batch_z(const double *A, int64_t count, int n,
+                                  int ld, int64_t stride, int32_t *piv,
+                                  double *gap, int32_t *info, int nthreads) {
+    set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < count; ++m) {
+        int r = crout_ref_z_gap(A + 2 * m * stride, n, ld, NULL, NULL, NULL,
+                                piv + m * n, gap + m * n);
         if (info) info[m] = r;
     }
 }

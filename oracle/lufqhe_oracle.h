@@ -57,6 +57,16 @@ void oracle_crout_ref_batch_z(const double *A, int64_t count, int n, int ld,
                               int64_t stride, double *det_plain, double *mant,
                               int32_t *exp2, int32_t *info, int32_t *piv,
                               int nthreads);
+/* Pivot sequence of the restated Crout with, per step j, the relative margin
+ * (c_best - c_second) / c_best of the chosen criterion over the best other
+ * candidate row (0 for an exact tie, +inf at the last step).  piv and gap
+ * hold n entries per matrix. */
+void oracle_crout_ref_gap_batch_z(const double *A, int64_t count, int n,
+                                  int ld, int64_t stride, int32_t *piv,
+                                  double *gap, int32_t *info, int nthreads);
+/* The fused CROUT_REF criterion of the kernel, x / s from y = RN(1/s) (see
+ * lup_mirror): equal to RN(x / s). */
+double oracle_quot_rn(double x, double s, double y);
 void oracle_crout_ref_batch_c(const float *A, int64_t count, int n, int ld,
                               int64_t stride, float *det_plain, int32_t *info,
                               int nthreads);

@@ -36,6 +36,10 @@ def load() -> ctypes.CDLL:
         lib.oracle_lup_mirror_mixed_z.argtypes = [_vp, _i64, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _i32]
         lib.oracle_lup_ld_mixed_z.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32]
         lib.oracle_crout_ref_pairsum_z.argtypes = [_vp, _i64, _i32, _i32, _vp]
+        lib.oracle_crout_ref_gap_batch_z.argtypes = [_vp, _i64, _i32, _i32, _i64, _vp, _vp, _vp,
+                                                     _i32]
+        lib.oracle_quot_rn.restype = ctypes.c_double
+        lib.oracle_quot_rn.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         _lib = lib
     return _lib
 
@@ -64,6 +68,43 @@ def crout_ref(A, nthreads: int = 0, pivots: bool = False):
     load().oracle_crout_ref_batch_z(_p(A), count, n, n, n * n, _p(det), _p(mant), _p(exp),
                                     _p(info), _p(piv), nthreads)
     return dict(det=det, mant=mant, exp=exp, info=info, piv=piv)
+
+
+def crout_ref_gaps(A, nthreads: int = 0):
+    """Pivot sequence of the restated Crout and, per step, the relative margin
+    (c_best - c_second) / c_best of its choice: dict(piv, gap, info), piv and
+    gap of shape (count, n); gap is 0 at an exact tie, inf at the last step."""
+    A, count, n = _batch(A, np.complex128)
+    piv = np.empty((count, n), np.int32)
+    gap = np.empty((count, n), np.float64)
+    info = np.empty(count, np.int32)
+    load().oracle_crout_ref_gap_batch_z(_p(A), count, n, n, n * n, _p(piv), _p(gap), _p(info),
+                                        nthreads)
+    return dict(piv=piv, gap=gap, info=info)
+
+
+def pivot_parity(piv_gpu, ref: dict, margin: float) -> dict:
+    """Compares a pivot table with crout_ref_gaps' output.  A matrix whose
+    sequences differ is an excluded near-tie when the reference's margin at
+    the first differing step is below `margin`, else a mismatch."""
+    piv_gpu = np.asarray(piv_gpu, np.int32)
+    ok = ref["info"] == 0
+    diff = (piv_gpu != ref["piv"]) & ok[:, None]
+    rows = np.nonzero(diff.any(axis=1))[0]
+    first = diff[rows].argmax(axis=1)
+    near = ref["gap"][rows, first] < margin
+    ties = int((ref["gap"][ok] == 0).any(axis=1).sum())
+    return {"matrices": int(len(piv_gpu)), "compared": int(ok.sum()),
+            "differing": int(len(rows)), "excluded_near_ties": int(near.sum()),
+            "mismatches": int((~near).sum()), "margin": margin,
+            "matrices_with_exact_ties": ties,
+            "mismatch_index": [int(i) for i in rows[~near][:10]]}
+
+
+def quot_rn(x: float, s: float) -> float:
+    """The kernel's fused CROUT_REF criterion x / s (corrected product)."""
+    y = 1.0 / s if s <= 2.0 ** 960 else float("nan")
+    return load().oracle_quot_rn(x, s, y)
 
 
 def crout_ref_c64(A, nthreads: int = 0):

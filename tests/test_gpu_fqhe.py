@@ -176,3 +176,33 @@ def test_fqhe_wrapper_validation(lc, dev):
         lc.fqhe_det(z, row_perm=torch.zeros(3, 8, dtype=torch.int64, device=dev), count=3)
     with pytest.raises(ValueError):
         lc.fqhe_det(torch.zeros(4, 8, dtype=torch.complex128, device=dev), count=3)
+
+
+def test_out_of_range_permutation_entry_is_reported(lc, wl, dev):
+    """A row-permutation entry >= n is an error of that matrix, not a
+    clamped row: its determinant outputs are NaN with info -1000, the other
+    matrices of the launch are untouched, and the materialised build writes
+    that row as NaN."""
+    n, count = 12, 40
+    z = torch.from_numpy(positions(wl, 1, n, 4)).to(dev)
+    rng = np.random.default_rng(8)
+    perm = np.stack([rng.permutation(n) for _ in range(count)]).astype(np.uint8)
+    good = lc.fqhe_det(z, row_perm=torch.from_numpy(perm).to(dev), count=count)
+    bad_perm = perm.copy()
+    bad_perm[3, 5] = n          # one past the end
+    bad_perm[17, 0] = 255
+    res = lc.fqhe_det(z, row_perm=torch.from_numpy(bad_perm).to(dev), count=count, pivots=True)
+    torch.cuda.synchronize()
+    info = res.info.cpu().numpy()
+    assert info[3] == -1000 and info[17] == -1000
+    keep = np.ones(count, bool)
+    keep[[3, 17]] = False
+    assert np.all(info[keep] == 0)
+    mant = res.mant.cpu().numpy()
+    assert np.isnan(mant[[3, 17]]).all() and np.isnan(res.logabs.cpu().numpy()[[3, 17]]).all()
+    assert np.array_equal(bits(mant[keep]), bits(good.mant.cpu().numpy()[keep]))
+    assert np.all(res.piv.cpu().numpy()[[3, 17]] == -1)
+    A = lc.fqhe_build(z.expand(count, n).contiguous(), row_perm=torch.from_numpy(bad_perm).to(dev))
+    A = A.cpu().numpy()
+    assert np.isnan(A[3, 5]).all() and np.isnan(A[17, 0]).all()
+    assert np.isfinite(A[keep]).all() and np.isfinite(np.delete(A[3], 5, axis=0)).all()

@@ -116,3 +116,90 @@ def test_server_validation(lc, dev):
     st, rec = srv.run(requests=0)
     assert st["requests"] == 0
     srv.close()
+
+
+def exact_sum(mant, exp, e0):
+    """sum mant_t * 2^(exp_t - e0) as exact Fractions (re, im), and sum |det_t| * 2^-e0."""
+    from fractions import Fraction
+    re = im = Fraction(0)
+    for m, e in zip(mant, exp):
+        s = Fraction(2) ** (int(e) - e0)
+        re += Fraction(float(m.real)) * s
+        im += Fraction(float(m.imag)) * s
+    mag = float(np.sum(np.abs(mant) * np.exp2(exp.astype(np.float64) - e0)))
+    return re, im, mag
+
+
+def test_serving_config_full_size_against_oracle(lc, orc, wl, dev):
+    """The serving configuration at its real size: 512 walkers (n = 8, 16, 24,
+    32, 48, 64 in turn), 2000 permutations per walker, k ~ U[1, 2000] terms
+    per request, as in the reference's Monte Carlo iteration
+    (LU-fqhe/src/MonteCarloLoop.cu:280-400).
+      * every record is bit-identical served alone (one request per batch, one
+        batch in flight) and in the mix (Poisson arrivals near capacity, up to
+        128 requests per batch, 4 batches in flight);
+      * for 3 requests of each size (among them the largest k of each n),
+        replaying the walker's trajectory: psi' equals the exact sum of the
+        oracle mirror's determinants of the permuted FQHE matrices (as the
+        device builds them) to 1e-12 of sum |det_t|, and the long-double
+        truth of the host-built matrices to 1e-9 of it."""
+    from fractions import Fraction
+    walkers, perms = 512, 2000
+    wn = [SIZES[w % len(SIZES)] for w in range(walkers)]
+    srv = lc.Server(wn, perms_per_walker=perms, seed=7)
+    z = {w: srv.walker(w) for w in range(walkers)}
+    tables = {w: srv.perms(w) for w in range(walkers)}
+    common = dict(requests=1536, k_min=1, k_max=2000, seed=13, hbm_budget=200 * 10 ** 9)
+    st_a, alone = srv.run(rate=0.0, max_batch_requests=1, max_inflight=1, **common)
+    srv.reset()
+    st_m, mix = srv.run(rate=40000.0, max_batch_requests=128, max_inflight=4, **common)
+    print(f"alone: {st_a['dets_per_s']:.3e} det/s; mix: {st_m['dets_per_s']:.3e} det/s, "
+          f"p50 {st_m['p50_s'] * 1e3:.3f} ms, p99 {st_m['p99_s'] * 1e3:.3f} ms, "
+          f"{st_m['batches']} batches", flush=True)
+    assert st_a["batches"] == 1536 and st_m["batches"] < 1536   # requests shared batches
+    assert_same(alone, mix)
+    assert int(alone["k"].max()) > 1900 and int(alone["k"].min()) < 100
+
+    # requests to check: per n, the largest k and two others
+    pick = set()
+    for n in SIZES:
+        rs = [r for r in range(len(mix)) if wn[int(mix["walker"][r])] == n]
+        rs.sort(key=lambda r: -int(mix["k"][r]))
+        pick.update([rs[0], rs[len(rs) // 2], rs[-1]])
+    worst_mirror = worst_truth = 0.0
+    for r in range(len(mix)):
+        rec = mix[r]
+        w = int(rec["walker"])
+        zp = z[w].copy()
+        zp[rec["particle"]] += rec["delta"]
+        if r in pick:
+            n, k = wn[w], int(rec["k"])
+            idx = (int(rec["perm_start"]) + np.arange(k)) % perms
+            perm = np.ascontiguousarray(tables[w][idx])
+            A = lc.fqhe_build(torch.from_numpy(zp[None]).to(dev),
+                              row_perm=torch.from_numpy(perm).to(dev), count=k).cpu().numpy()
+            o = orc.mirror(A, "crout_ref", "fused", pivots=False)
+            assert np.all(o["info"] == 0)
+            e0 = int(o["exp"].max())
+            re, im, mag = exact_sum(o["mant"], o["exp"], e0)
+            got_re = Fraction(float(rec["sum_mant"].real)) * Fraction(2) ** (int(rec["sum_exp"]) - e0)
+            got_im = Fraction(float(rec["sum_mant"].imag)) * Fraction(2) ** (int(rec["sum_exp"]) - e0)
+            err = math.hypot(float(got_re - re), float(got_im - im)) / mag
+            worst_mirror = max(worst_mirror, err)
+            assert err <= 1e-12, (r, n, k, err)
+            # accuracy: long-double truth of the host-built matrices
+            base = wl.fqhe_matrix_seq(zp)
+            t = orc.truth(base[perm])
+            d = np.exp((t["logabs"] - t["logabs"].max()) + 1j * t["phase"])
+            scale = math.log(2.0) * int(rec["sum_exp"]) - float(t["logabs"].max())
+            psi = complex(rec["sum_mant"]) * math.exp(scale)
+            errt = abs(psi - d.sum()) / np.abs(d).sum()
+            worst_truth = max(worst_truth, errt)
+            assert errt <= 1e-9, (r, n, k, errt)
+        if rec["accepted"]:
+            z[w] = zp
+    print(f"checked {len(pick)} requests: worst vs mirror {worst_mirror:.2e}, "
+          f"vs long-double truth {worst_truth:.2e} (relative to sum |det_t|)", flush=True)
+    for w in range(walkers):
+        assert np.array_equal(bits(srv.walker(w)), bits(z[w]))
+    srv.close()

@@ -164,11 +164,133 @@ def test_pivot_rule_is_scaled_real_part_with_last_tie(orc):
     B = np.array([[1 + 0j, 0.5], [2, 1]], complex)
     out = orc.crout_ref(B, pivots=True)
     assert out["piv"][0, 0] == 1
-    # ABS policy uses the modulus: row 1 (|1+9i|) wins column 0 of A
+    # ABS policy uses |Re| + |Im| (LAPACK's cabs1): row 1 (1 + 9) wins column 0 of A
     m = orc.mirror(A, "abs", "fused")
     assert m["piv"][0, 0] == 1
     m2 = orc.mirror(np.array([[3 + 0j, 1], [0 + 3j, 1]]), "abs", "fused")
-    assert m2["piv"][0, 0] == 1  # |3| == |3i|: tie to the last row
+    assert m2["piv"][0, 0] == 0  # |3| + 0 == 0 + |3|: tie to the first row, as izamax
+    m3 = orc.mirror(np.array([[2 + 2j, 1], [0 + 3j, 1]]), "abs", "fused")
+    assert m3["piv"][0, 0] == 0  # 4 > 3 although |2+2i| < |3i|: cabs1, not the modulus
+
+
+def test_fused_criterion_keeps_non_power_of_two_ties(orc):
+    """Both rows of column 0 have the criterion 1 exactly (their max |Re| is in
+    column 0), so the reference's `>=` picks the last row.  49 * RN(1/49)
+    rounds to 1 - 2^-53, so a product criterion would pick row 0; the fused
+    mirror's corrected quotient keeps the tie."""
+    assert 49.0 * (1.0 / 49.0) < 1.0
+    A = np.array([[3 + 0j, 2], [49 + 1j, 1]], complex)
+    assert orc.crout_ref(A, pivots=True)["piv"][0, 0] == 1
+    assert orc.mirror(A, "crout_ref", "fused")["piv"][0, 0] == 1
+    # the same tie inside a larger matrix, at several non-power-of-two scales
+    rng = np.random.default_rng(12)
+    for x in (49.0, 0.1, 3.7e-5, 98.3, 1e20 / 3):
+        B = rng.standard_normal((9, 9)) * 0.01 + 0j
+        B[:, 1] = 1.0     # every other row's largest real part is off column 0
+        B[2, 0] = x
+        B[2, 1:] = x / 7
+        B[6, 0] = 7.0
+        B[6, 1:] = 1.0
+        cr = orc.crout_ref(B, pivots=True)
+        assert cr["piv"][0, 0] == 6
+        assert np.array_equal(orc.mirror(B, "crout_ref", "fused")["piv"], cr["piv"])
+
+
+def _hard_quotients(count, seed):
+    """(x, s) in [1, 2) whose quotient x/s lies within 6 * 2^-106 of a
+    rounding midpoint: B*M = A*2^54 - d for small d, M odd (the hardest cases
+    for correct rounding of a division)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        M = int(rng.integers(2 ** 53, 2 ** 54)) | 1
+        d = int(rng.integers(1, 7)) * (1 if rng.random() < 0.5 else -1)
+        B = (-d * pow(M, -1, 2 ** 54)) % 2 ** 54
+        if not 2 ** 52 <= B < 2 ** 53:
+            continue
+        A, rem = divmod(B * M + d, 2 ** 54)
+        assert rem == 0
+        if A >= 2 ** 53:
+            continue
+        out.append((A * 2.0 ** -52, B * 2.0 ** -52))
+    return out
+
+
+def test_corrected_quotient_is_the_division(orc):
+    """The fused criterion's corrected product equals the IEEE division on
+    the hardest-to-round quotients (where the uncorrected product is wrong
+    about half the time), on random quotients over many binades, at the
+    edges of its fast range and on its fallback inputs."""
+    hard = _hard_quotients(4000, 3)
+    wrong_product = 0
+    for x, s in hard:
+        assert orc.quot_rn(x, s) == x / s, (x.hex(), s.hex())
+        wrong_product += (x * (1.0 / s)) != x / s
+    assert wrong_product > 1000
+    rng = np.random.default_rng(4)
+    xs = np.ldexp(rng.random(20000) + 0.5, rng.integers(-300, 300, 20000))
+    ss = np.ldexp(rng.random(20000) + 0.5, rng.integers(-80, 300, 20000))
+    for x, s in zip(xs.tolist(), ss.tolist()):
+        assert orc.quot_rn(x, s) == x / s
+    for x, s in ((0.0, 3.0), (5e-324, 1e-24), (2.0 ** -900, 1.0), (2.0 ** 1000, 2.0 ** -70),
+                 (1.5 * 2.0 ** 1023, 2.0 ** 1000), (3.0, 2.0 ** 1020), (math.inf, 2.0),
+                 (2.0 ** -879, 1.0), (2.0 ** -881, 1.0), (2.0 ** 959, 1.0), (2.0 ** 961, 1.0)):
+        assert orc.quot_rn(x, s) == x / s, (x, s)
+    assert math.isnan(orc.quot_rn(math.nan, 2.0))
+
+
+@pytest.mark.parametrize("n,count", [(16, 4096), (32, 65536)])
+def test_fused_pivots_equal_reference_crout(orc, wl, n, count):
+    """The fused mirror (the GPU kernel's operation order) picks the
+    restated reference Crout's pivot at every step of every matrix of the
+    4096 x 16x16 and 65536 x 32x32 batches, exact criterion ties included:
+    only the arithmetic of the eliminations differs, and no step comes within
+    1e-12 of a tie between different values."""
+    A = wl.gaussian_np(count, n, seed=77 + n)
+    r = orc.pivot_parity(orc.mirror(A, "crout_ref", "fused")["piv"], orc.crout_ref_gaps(A), 1e-12)
+    assert r["mismatches"] == 0 and r["differing"] == 0, r
+    assert r["matrices_with_exact_ties"] > count // 8   # ties at 1.0 in column 0 are common
+
+
+def test_fused_pivots_equal_reference_crout_mixed_and_fqhe(orc, wl):
+    """The same on a mixed n = 8..64 sample with 5% near-singular matrices
+    and on FQHE matrices (whose column 0 is exp(-|z|^2/4), often the row's
+    largest real part)."""
+    flat, ns, offs, _ = wl.mixed_batch(3000, 2024)
+    flat = flat.numpy() if hasattr(flat, "numpy") else flat
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        A = np.stack([flat[offs[i]:offs[i] + n * n].reshape(n, n) for i in idx])
+        r = orc.pivot_parity(orc.mirror(A, "crout_ref", "fused")["piv"], orc.crout_ref_gaps(A),
+                             1e-12)
+        assert r["mismatches"] == 0, (int(n), r)
+    rng = np.random.default_rng(31)
+    for n in (10, 20, 33, 40):
+        A = np.stack([wl.fqhe_matrix(wl.fqhe_positions(n, rng)) for _ in range(200)])
+        r = orc.pivot_parity(orc.mirror(A, "crout_ref", "fused")["piv"], orc.crout_ref_gaps(A),
+                             1e-12)
+        assert r["mismatches"] == 0, (n, r)
+
+
+@pytest.mark.parametrize("policy,ks", [("crout_ref", (60, 400, 515, 600, 900, 1000)),
+                                       ("abs", (-900, -600, -515, 60, 515, 600, 900, 1000))])
+def test_power_of_two_scaling_is_exact(orc, wl, policy, ks):
+    """det(2^k A) = 2^(k n) det(A) exactly: the fused mirror returns the same
+    mantissa bits and pivots, and the exponent moves by k*n, including where
+    |p|^2 of an unscaled pivot would overflow (k >= 512) or underflow
+    (k <= -512) and where the row scale leaves the corrected quotient's fast
+    range (k = 1000).  (CROUT_REF rejects rows of max |Re| < 1e-24, so it
+    takes no negative k.)"""
+    for n in (3, 16, 33, 64):
+        base = wl.gaussian_np(16, n, seed=500 + n)
+        b = orc.mirror(base, policy, "fused")
+        assert np.all(b["info"] == 0)
+        for k in ks:
+            m = orc.mirror(base * 2.0 ** k, policy, "fused")
+            assert np.all(m["info"] == 0), (n, k)
+            assert np.array_equal(bits(m["mant"]), bits(b["mant"])), (n, k)
+            assert np.array_equal(m["exp"], b["exp"] + k * n), (n, k)
+            assert np.array_equal(m["piv"], b["piv"]), (n, k)
 
 
 def test_status_codes(orc):
