@@ -77,6 +77,8 @@ def _run(cmd: list[str], verbose: bool) -> None:
 
 def _compile_jobs() -> list[tuple[Path, list[str], list[Path]]]:
     hdrs = _headers()
+    # the kernel instantiation units include only the kernel headers
+    kern_hdrs = sorted(CSRC.glob("*.hpp"))
     jobs = []
     # the persistent class kernels are the longest single compiles: first
     for v in range(N_PERSIST_VARIANTS):
@@ -84,12 +86,13 @@ def _compile_jobs() -> list[tuple[Path, list[str], list[Path]]]:
             obj = BUILD / f"lup_persist_v{v}_c{c}.o"
             defs = [f"-DLUFQHE_VARIANT={v}", f"-DLUFQHE_CLASS={c}"]
             jobs.append((obj, defs + ["-c", str(CSRC / "lup_persist_inst.hip")],
-                         hdrs + [CSRC / "lup_persist_inst.hip"]))
+                         kern_hdrs + [CSRC / "lup_persist_inst.hip"]))
     for v in range(N_VARIANTS):
         for s in range(N_SHARDS):
             obj = BUILD / f"lup_inst_v{v}_s{s}.o"
             defs = [f"-DLUFQHE_VARIANT={v}", f"-DLUFQHE_SHARD={s}", f"-DLUFQHE_SHARDS={N_SHARDS}"]
-            jobs.append((obj, defs + ["-c", str(CSRC / "lup_inst.hip")], hdrs + [CSRC / "lup_inst.hip"]))
+            jobs.append((obj, defs + ["-c", str(CSRC / "lup_inst.hip")],
+                         kern_hdrs + [CSRC / "lup_inst.hip"]))
     for src in sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp")):
         if src.name in ("lup_inst.hip", "lup_persist_inst.hip"):
             continue
