@@ -47,6 +47,7 @@ extern "C" {
 #define LUFQHE_ERROR_NOT_BUILT 4
 #define LUFQHE_ERROR_OUT_OF_BUDGET 5
 #define LUFQHE_ERROR_IO 6
+#define LUFQHE_ERROR_TOO_LARGE 7
 
 /* Pivot policies.  CROUT_REF is the reference's rule: argmax over remaining
  * rows of the correctly rounded |Re u_ik| / s_i, s_i = max_j |Re a_ij| of the
@@ -121,6 +122,20 @@ int64_t lufqhe_det_inplace_workspace_bytes(int64_t count);
 int32_t lufqhe_det_inplace(void *d_A, int64_t count, int32_t n, int64_t ld,
                            int64_t stride, int32_t pair_product,
                            int32_t policy, void *d_work, hipStream_t stream);
+
+/* No counterpart in the reference: lufqhe_det_inplace for callers that
+ * cannot allocate (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  The same result, with the workspace in a fixed device array
+ * of LUFQHE_STATIC_MAX_MATRICES * 20 bytes (20 MiB) that each GPU holds
+ * until the process ends; batches above LUFQHE_STATIC_MAX_MATRICES matrices
+ * return LUFQHE_ERROR_TOO_LARGE.  The first call on a device looks up the
+ * array's address (hipGetSymbolAddress); no call allocates.  Calls on one
+ * GPU share its array: calls that overlap on one GPU race. */
+#define LUFQHE_STATIC_MAX_MATRICES (1 << 20)
+int32_t lufqhe_det_inplace_static(void *d_A, int64_t count, int32_t n,
+                                  int64_t ld, int64_t stride,
+                                  int32_t pair_product, int32_t policy,
+                                  hipStream_t stream);
 
 /* Bytes of device workspace the two pair sums need (fixed). */
 int64_t lufqhe_pair_sum_workspace_bytes(void);

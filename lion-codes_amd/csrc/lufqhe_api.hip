@@ -91,6 +91,7 @@ const char* lufqhe_status_string(int32_t status) {
     case LUFQHE_ERROR_NOT_BUILT: return "kernel variant not built";
     case LUFQHE_ERROR_OUT_OF_BUDGET: return "memory budget exceeded";
     case LUFQHE_ERROR_IO: return "file cannot be read";
+    case LUFQHE_ERROR_TOO_LARGE: return "batch exceeds the fixed workspace of the static entry point";
     default: return "unknown status";
   }
 }

@@ -31,6 +31,7 @@ ERROR_HIP = 3
 ERROR_NOT_BUILT = 4
 ERROR_OUT_OF_BUDGET = 5
 ERROR_IO = 6
+ERROR_TOO_LARGE = 7
 FILE_FORMATS = {"complex64": 0, "complex128": 1}
 
 _lib = None
@@ -70,6 +71,8 @@ def load() -> ctypes.CDLL:
     lib.lufqhe_det_inplace_workspace_bytes.argtypes = [_i64]
     lib.lufqhe_det_inplace.restype = _i32
     lib.lufqhe_det_inplace.argtypes = [_vp, _i64, _i32, _i64, _i64, _i32, _i32, _vp, _vp]
+    lib.lufqhe_det_inplace_static.restype = _i32
+    lib.lufqhe_det_inplace_static.argtypes = [_vp, _i64, _i32, _i64, _i64, _i32, _i32, _vp]
     lib.lufqhe_pair_sum_workspace_bytes.restype = _i64
     lib.lufqhe_pair_sum_workspace_bytes.argtypes = []
     lib.lufqhe_global_reduction.restype = _i32
