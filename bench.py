@@ -165,6 +165,26 @@ def cpu_checks(out: dict, samples: dict) -> None:
         A, la, ph, piv = samples["small"]
         out["small_batch_n16"]["accuracy"] = dict(summarize([parity_report(A, la, ph, piv)]),
                                                   scope=f"all {len(A)} matrices")
+    if "fqhe" in samples:
+        from oracle import oracle
+        groups, la_ref, ph_ref, logs = samples["fqhe"]
+        la_t = np.empty(len(la_ref))
+        ph_t = np.empty(len(la_ref))
+        for idx, A in groups:
+            t = oracle.truth(A)
+            la_t[idx], ph_t[idx] = t["logabs"], t["phase"]
+
+        def rel(la, ph, lr, pr):
+            return np.abs((la - lr) + 1j * np.angle(np.exp(1j * (ph - pr)))) / np.abs(lr + 1j * pr)
+        floor = rel(la_t, ph_t, la_ref, ph_ref)
+        out["fqhe"]["long_double_floor_vs_analytic"] = {
+            "what": "a long-double LU of the same FP64 matrices against the analytic form",
+            "count_gt_1e-10": int((floor > 1e-10).sum()), "max": float(floor.max())}
+        for policy, (la, ph) in logs.items():
+            e = rel(la, ph, la_t, ph_t)
+            out["fqhe"][policy]["vs_long_double_same_matrices"] = {
+                "median": float(np.median(e)), "max": float(e.max()),
+                "count_gt_1e-10": int((e > 1e-10).sum())}
     if "mixed" in samples:
         groups = samples["mixed"]
         parts = [parity_report(A, la, ph, piv) for (A, la, ph, piv, _) in groups]
@@ -292,38 +312,80 @@ def bench_mixed(lc, wl, dev, reps: int, samples: dict) -> dict:
     return out
 
 
-def bench_fqhe(lc, wl, dev, reps: int) -> dict:
+def bench_fqhe(lc, wl, dev, reps: int, samples: dict) -> dict:
     """FQHE batch: 32768 FQHE matrices, n ~ U{10..40}, built on the device from
-    particle positions inside the determinant kernels."""
+    particle positions inside the determinant kernels; timed under the
+    reference's pivot rule, and run under both pivot policies for accuracy
+    against the analytic form (and, in the CPU leg, against a long-double LU
+    of the same device-built FP64 matrices)."""
     import torch
     count, seed = 32768, 3
     z, ns, offs = wl.fqhe_configs(count, 10, 40, seed=seed)
     zd = torch.from_numpy(z).to(dev)
-    plan = lc.MixedPlan(ns, offs, nstreams=4)
-    res = lc.alloc_result(count, 64, dev)
-    side = torch.cuda.Stream()
-    with torch.cuda.stream(side):
-        plan.execute(zd, res, mode="graph", stream=side, source="fqhe")
-        ms = time_events(lambda: plan.execute(zd, res, mode="graph", stream=side, source="fqhe"),
-                         side, reps)
-    side.synchronize()
     la_ref = np.empty(count)
     ph_ref = np.empty(count)
     for m in range(count):
         la_ref[m], ph_ref[m] = wl.fqhe_analytic(z[offs[m]:offs[m] + ns[m]])
-    la = res.logabs.cpu().numpy()
-    ph = res.phase.cpu().numpy()
-    dph = np.angle(np.exp(1j * (ph - ph_ref)))
-    rel = np.abs((la - la_ref) + 1j * dph) / np.abs(la_ref + 1j * ph_ref)
+    out = {"workload": f"{count} FQHE matrices, n ~ U{{10..40}}, z uniform in the disk of "
+                       f"radius sqrt(2n), A_ij = z_i^j exp(-|z_i|^2/4) built in the kernel, "
+                       f"seed {seed}"}
     total_flops = float(np.sum([det_flops(int(n)) for n in ns]))
-    plan.close()
-    return {"workload": f"{count} FQHE matrices, n ~ U{{10..40}}, z uniform in the disk of "
-                        f"radius sqrt(2n), A_ij = z_i^j exp(-|z_i|^2/4) built in the kernel, "
-                        f"seed {seed}",
-            "ms": ms, "det_per_s": count / (ms * 1e-3),
-            "fp64_frac": total_flops / (ms * 1e-3) / (FP64_PEAK_TFLOPS * 1e12),
-            "analytic_log_form": {"median": float(np.median(rel)), "max": float(rel.max()),
-                                  "frac_le_1e-10": float(np.mean(rel <= 1e-10))}}
+    side = torch.cuda.Stream()
+    logs = {}
+    for policy in ("crout_ref", "abs"):
+        plan = lc.MixedPlan(ns, offs, policy=policy, nstreams=4)
+        res = lc.alloc_result(count, 64, dev)
+        with torch.cuda.stream(side):
+            plan.execute(zd, res, mode="graph", stream=side, source="fqhe")
+            ms = time_events(lambda: plan.execute(zd, res, mode="graph", stream=side,
+                                                  source="fqhe"), side, reps)
+        side.synchronize()
+        la = res.logabs.cpu().numpy()
+        ph = res.phase.cpu().numpy()
+        logs[policy] = (la, ph)
+        rel = np.abs((la - la_ref) + 1j * np.angle(np.exp(1j * (ph - ph_ref)))) / np.abs(
+            la_ref + 1j * ph_ref)
+        entry = {"ms": ms, "det_per_s": count / (ms * 1e-3),
+                 "fp64_frac": total_flops / (ms * 1e-3) / (FP64_PEAK_TFLOPS * 1e12),
+                 "vs_analytic": {"median": float(np.median(rel)), "max": float(rel.max()),
+                                 "frac_le_1e-10": float(np.mean(rel <= 1e-10)),
+                                 "count_gt_1e-10": int((rel > 1e-10).sum())}}
+        plan.close()
+        if policy == "crout_ref":
+            out.update({k: entry[k] for k in ("ms", "det_per_s", "fp64_frac")})
+        out[policy] = entry
+    # the device-built matrices, per n, for the CPU leg's long-double LU
+    groups = []
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        zz = torch.from_numpy(np.stack([z[offs[m]:offs[m] + n] for m in idx])).to(dev)
+        groups.append((idx, lc.fqhe_build(zz).cpu().numpy()))
+    samples["fqhe"] = (groups, la_ref, ph_ref, logs)
+    return out
+
+
+def bench_sizes(lc, wl, dev, reps: int) -> dict:
+    """Per-size uniform batches of about 256 MiB (n = 8, 16, ..., 64), one
+    plain launch each with the tuned shape: ns per determinant and the
+    fraction of the HBM and FP64 roofs, per n."""
+    import torch
+    out = {}
+    stream = torch.cuda.current_stream()
+    for n in (8, 16, 24, 32, 40, 48, 56, 64):
+        count = (256 << 20) // (16 * n * n)
+        A = wl.gaussian(count, n, 500 + n, device=dev)
+        res = lc.alloc_result(count, n, dev)
+        lc.det_batched(A, out=res)
+        ms = time_events(lambda: lc.det_batched(A, out=res), stream, reps)
+        assert int((res.info != 0).sum()) == 0
+        lanes, cols, waves = lc.shape_info(n, lc.get_shape("crout_ref", "fused", n))
+        out[str(n)] = {"count": count, "ms": ms, "ns_per_det": ms * 1e6 / count,
+                       "shape": {"lanes": lanes, "lane_cols": cols, "waves": waves},
+                       "hbm_frac": det_bytes(n) * count / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                       "fp64_frac": det_flops(n) * count / (ms * 1e-3) / (FP64_PEAK_TFLOPS * 1e12)}
+        del A, res
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_serving(lc, dev) -> dict:
@@ -371,8 +433,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the mixed-size, FQHE and serving measurements")
-    ap.add_argument("--legs", default="mixed,fqhe,serving",
-                    help="comma list of extra measurements to run (mixed, fqhe, serving)")
+    ap.add_argument("--legs", default="sizes,mixed,fqhe,serving",
+                    help="comma list of extra measurements to run (sizes, mixed, fqhe, serving)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -470,12 +532,15 @@ def main() -> None:
         del A
         torch.cuda.empty_cache()
         reps = max(3, min(args.steps, 10))
+    if "sizes" in legs:
+        print("# per-size batches", file=sys.stderr, flush=True)
+        out["sizes"] = bench_sizes(lc, wl, dev, reps)
     if "mixed" in legs:
         print("# mixed-size batch", file=sys.stderr, flush=True)
         out["mixed"] = bench_mixed(lc, wl, dev, reps, samples)
     if "fqhe" in legs:
         print("# FQHE batch", file=sys.stderr, flush=True)
-        out["fqhe"] = bench_fqhe(lc, wl, dev, reps)
+        out["fqhe"] = bench_fqhe(lc, wl, dev, reps, samples)
     if "serving" in legs:
         print("# walker serving", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(lc, dev)

@@ -301,6 +301,32 @@ int32_t lufqhe_read_matrices(const char *path, int32_t format, int64_t count,
                              void *d_out, int64_t capacity, int64_t *read_out,
                              hipStream_t stream);
 
+/* ---- the reference's Monte Carlo loop, LU mode, on one GPU ---------------
+ * MonteCarloLoop::perform (LU-fqhe/src/MonteCarloLoop.cu:121-401) for one
+ * node with one GPU: the node's nelem matrices (host memory, complex128,
+ * n x n row major, packed) are split by the reference's rule -- one GPU
+ * counts as 32 host cores and takes an even share, floor(nelem / (1 +
+ * cpu_count/32)) rounded down to even (LU-fqhe/src/MonteCarloLoop.cu:139-148,
+ * lufqhe_mc_split) -- and each of `iterations` iterations copies the GPU
+ * share to the device, runs lufqhe_det_inplace (pair products) and
+ * lufqhe_global_reduction there, while the host cores sum their share's
+ * pair products (lufqhe_cpu_pair_sum, LUCoreCPU::CALL); h_results[i]
+ * (complex128) gets the node's sum, GPU plus host (LU-fqhe/src/
+ * MonteCarloLoop.cu:360-364).  The GPU share is staged once in pinned memory,
+ * so each iteration's copy overlaps the host work.  Allocates its device and
+ * pinned buffers for the call.  A host share holding a row the reference
+ * would exit(0) on gives NaN results and LUFQHE_ERROR_INVALID_VALUE. */
+typedef struct {
+  int64_t gpu_matrices, cpu_matrices; /* the split */
+  double wall_s, gpu_s, cpu_s;        /* totals over the iterations */
+} lufqhe_mc_stats;
+
+int32_t lufqhe_mc_split(int64_t nelem, int32_t cpu_count, int64_t *gpu_matrices,
+                        int64_t *cpu_matrices);
+int32_t lufqhe_mc_lu_run(const void *h_matrices, int64_t nelem, int32_t n,
+                         int32_t cpu_count, int32_t iterations, int32_t policy,
+                         void *h_results, lufqhe_mc_stats *stats);
+
 /* ---- serving: determinants for Monte Carlo walkers ----------------------
  * The reference's Monte Carlo loop (LU-fqhe/src/MonteCarloLoop.cu:280-400)
  * rebuilds all matrices on the host every iteration and runs one blocking

@@ -22,6 +22,8 @@ from .lucore import (  # noqa: F401
     get_shape,
     global_reduction,
     load,
+    mc_lu_run,
+    mc_split,
     pair_sum,
     read_matrices,
     set_shape,

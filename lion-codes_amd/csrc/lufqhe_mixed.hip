@@ -53,6 +53,10 @@ struct lufqhe_mixed_plan {
   hipStream_t streams[LUFQHE_MIXED_MAX_STREAMS] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[LUFQHE_MIXED_MAX_STREAMS] = {};
+  // recorded after every persistent replay: the per-bucket arguments those
+  // replays read are rewritten only once it has completed
+  hipEvent_t replayed = nullptr;
+  bool replay_pending = false;
   // persistent mode: per class, its buckets' queue and launch size
   struct PClass {
     int cls = 0;
@@ -89,6 +93,7 @@ struct lufqhe_mixed_plan {
       if (streams[s]) (void)hipStreamDestroy(streams[s]);
     }
     if (fork) (void)hipEventDestroy(fork);
+    if (replayed) (void)hipEventDestroy(replayed);
     if (d_tables) (void)hipFree(d_tables);
     if (d_persist) (void)hipFree(d_persist);
   }
@@ -317,6 +322,7 @@ int32_t lufqhe_mixed_plan_create(const int32_t* h_n, const int64_t* h_offs, int6
     err = hipMemcpy(p->d_tables, tables.data(), tables.size() * sizeof(int64_t),
                     hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+  if (err == hipSuccess) err = hipEventCreateWithFlags(&p->replayed, hipEventDisableTiming);
   for (int s = 0; s < nstreams && err == hipSuccess; ++s) {
     err = hipStreamCreateWithFlags(&p->streams[s], hipStreamNonBlocking);
     if (err == hipSuccess) err = hipEventCreateWithFlags(&p->join[s], hipEventDisableTiming);
@@ -395,8 +401,9 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan* plan, const void* d_A, void* d_m
       plan->graph = nullptr;
       plan->g_mode = -1;
       if (persistent && (plan->p_A != d_A || plan->p_src != src || !same_out(plan->p_out, out))) {
-        // earlier replays on `stream` may still read the old arguments
-        err = hipStreamSynchronize(stream);
+        // earlier replays, on whatever stream, may still read the old
+        // arguments: wait for the last one
+        if (plan->replay_pending) err = hipEventSynchronize(plan->replayed);
         if (err == hipSuccess) err = write_persistent_args(*plan, d_A, src, out);
       }
       if (err == hipSuccess) err = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
@@ -415,6 +422,10 @@ int32_t lufqhe_mixed_execute(lufqhe_mixed_plan* plan, const void* d_A, void* d_m
       }
     }
     if (err == hipSuccess) err = hipGraphLaunch(plan->exec, stream);
+    if (err == hipSuccess && persistent) {
+      err = hipEventRecord(plan->replayed, stream);
+      plan->replay_pending = err == hipSuccess;
+    }
   } else {
     return LUFQHE_ERROR_INVALID_VALUE;
   }

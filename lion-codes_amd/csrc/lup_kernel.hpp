@@ -182,6 +182,31 @@ __device__ __forceinline__ double quot_rn(double x, double s, double y, bool nee
   return q;
 }
 
+// RN(1/d) for the scaled |p_s|^2 of a pivot, d in [0.25, 2), and 0 for
+// d == 0 (a zero pivot) or d == inf: Newton steps from the hardware
+// reciprocal estimate and a final residual correction -- the operations of
+// the IEEE division 1.0 / d without its range scaling and special-case
+// fixup, which are the identity for d in [0.25, 2).  NaN gives NaN.
+__device__ __forceinline__ double recip_scaled(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  const double rem = fma(-d, r, 1.0);
+  const double q = fma(rem, r, r);
+  // class mask: -inf, -0, +0, +inf
+  return __builtin_amdgcn_class(d, (1 << 2) | (1 << 5) | (1 << 6) | (1 << 9)) ? 0.0 : q;
+}
+
+// min(a, b) without the NaN-quieting canonicalisation fmin adds; a NaN
+// operand yields the other.
+__device__ __forceinline__ double vmin_f64(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 // mant * 2^e renormalisation, exact (power-of-two scaling).
 __device__ __forceinline__ void renorm(double& x, double& y, int& e) {
   const double m = vmax_abs_f64(x, y);           // a NaN operand yields the other
@@ -344,7 +369,7 @@ __device__ __forceinline__ void search_stage(int j, SearchState& st, const doubl
         // invariant under power-of-two scaling
         c = fabs(v.x) + fabs(v.y);
       }
-      c = fmin(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
+      c = vmin_f64(c, kDblMax);  // NaN -> DBL_MAX: exactly one pivot always exists
       c = cand ? c : -1.0;
       const int key = tie_key<POLICY>(pos[t]);
       if (t == 0) {
@@ -372,25 +397,60 @@ __device__ __forceinline__ void pin(SearchState& st) {
   asm volatile("" : "+v"(st.cl), "+v"(st.pl), "+v"(st.cm), "+v"(st.pm)::"memory");
 }
 
-// a[t][s] -= l_t * u_s for one column slot s (4 FMAs per complex element).
-template <int ARITH, int R, int C>
-__device__ __forceinline__ void update_slot(double2 (&a)[R][C], const double (&lx)[R],
-                                            const double (&ly)[R], int s, double2 u) {
+// v_fma_f64 with source negation: d = -a*b + c, and d = a*b + c.  Written
+// out because the compiler's accumulating v_fmac form has no modifiers and
+// would need negated copies of the multipliers.
+__device__ __forceinline__ double fma_na(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+__device__ __forceinline__ double fma_pa(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
+// a[t][s] -= l_t * u_s for the column slots s0 .. s0+M-1 below C (4 FMAs per
+// complex element).  In fused arithmetic the first FMA of every element
+// (x - lx*ux, y - lx*uy) comes before the second (+ ly*uy, - ly*ux), so
+// dependent FMAs are never adjacent.
+template <int ARITH, int R, int C, int M>
+__device__ __forceinline__ void update_slots(double2 (&a)[R][C], const double (&lx)[R],
+                                             const double (&ly)[R], int s0,
+                                             const double2 (&u)[M]) {
+  if constexpr (ARITH == ARITH_FUSED) {
+    double xs[M][R], ys[M][R];
 #pragma unroll
-  for (int t = 0; t < R; ++t) {
-    double2& e = a[t][s];
-    if constexpr (ARITH == ARITH_FUSED) {
-      double x = fma(-lx[t], u.x, e.x);
-      x = fma(ly[t], u.y, x);
-      double y = fma(-lx[t], u.y, e.y);
-      y = fma(-ly[t], u.x, y);
-      e.x = x;
-      e.y = y;
-    } else {
-      const double tx = lx[t] * u.x - ly[t] * u.y;
-      const double ty = ly[t] * u.x + lx[t] * u.y;
-      e.x = e.x - tx;
-      e.y = e.y - ty;
+    for (int c = 0; c < M; ++c) {
+      if (s0 + c >= C) continue;
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        xs[c][t] = fma_na(lx[t], u[c].x, a[t][s0 + c].x);
+        ys[c][t] = fma_na(lx[t], u[c].y, a[t][s0 + c].y);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      if (s0 + c >= C) continue;
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        a[t][s0 + c].x = fma_pa(ly[t], u[c].y, xs[c][t]);
+        a[t][s0 + c].y = fma_na(ly[t], u[c].x, ys[c][t]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+      if (s0 + c >= C) continue;
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        double2& e = a[t][s0 + c];
+        const double tx = lx[t] * u[c].x - ly[t] * u[c].y;
+        const double ty = ly[t] * u[c].x + lx[t] * u[c].y;
+        e.x = e.x - tx;
+        e.y = e.y - ty;
+      }
     }
   }
 }
@@ -419,7 +479,10 @@ __device__ __forceinline__ int update_and_search(
   constexpr int NS = search_stages<L>();
   constexpr int CH = 4;
   const int s0 = (k + 1) / CG;
-  update_slot<ARITH>(a, lx, ly, s0, bc[cb + s0 * CG]);
+  {
+    const double2 u0[1] = {bc[cb + s0 * CG]};
+    update_slots<ARITH>(a, lx, ly, s0, u0);
+  }
   pin_slot(a, s0);
 
   SearchState st{-1.0, -1, -1.0, -1};
@@ -442,9 +505,7 @@ __device__ __forceinline__ int update_and_search(
       for (int c = 0; c < CH; ++c)
         if (c0 + CH + c < C) ub[(ci + 1) & 1][c] = bc[cb + (c0 + CH + c) * CG];
     }
-#pragma unroll
-    for (int c = 0; c < CH; ++c)
-      if (c0 + c < C) update_slot<ARITH>(a, lx, ly, c0 + c, ub[ci & 1][c]);
+    update_slots<ARITH>(a, lx, ly, c0, ub[ci & 1]);
 #pragma unroll
     for (int j = 0; j < NS; ++j)
       if (j * nch / NS == ci) search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, k + 1);
@@ -456,11 +517,315 @@ __device__ __forceinline__ int update_and_search(
   return tie_key<POLICY>(st.pm);
 }
 
-// Factorisation of one matrix held by a group as a[R][C] (see Tile).  Rows
-// never move; each row slot carries its reference position.  piv_rec[q]
-// receives the pivot position chosen at step gl + q*L.
+// Per row slot state of a lane: the row's reference position, whether it is
+// still a pivot candidate, and its row scale s_i with RN(1/s_i).
+template <int R>
+struct RowSlots {
+  int pos[R];
+  bool act[R];
+  double sc[R], rs[R];
+};
+
+// Running state of one factorisation: det = sg * (dx, dy) * 2^de, the first
+// zero pivot's status, and the position of the next pivot.
+struct StepState {
+  double dx, dy, sg;
+  int de, info, ppos;
+};
+
+// Compaction schedule.  Rows retire as they become pivots, so once the
+// active rows fit in R-1 row slots per lane-row, the group moves them there
+// (compact below): the remaining steps then update, search and select over
+// R-1 slots instead of R.  Compaction happens where the active rows first
+// fit, if at least kMinCompactSteps steps remain to pay for it.
+constexpr int kMinCompactSteps = 6;
+
+constexpr int compact_step(int N, int RG, int R) {
+  return (R > 1 && (R - 1) * RG >= kMinCompactSteps) ? N - (R - 1) * RG : N;
+}
+
+// LDS budget of one wave's compactions, in double2 (20 KiB: eight waves of
+// a CU then fit in its 160 KiB).  A compaction moves its columns in as many
+// rounds as it takes to stay within the budget or within the pivot-row
+// slots, whichever is larger.
+constexpr int kCompactBudget = 1280;
+
+// LDS a compaction at step K needs per group (double2 units) when it moves
+// its columns in `rounds` rounds: per row the round's columns plus one
+// padding element (rows on distinct banks), then the rows' scales and
+// reciprocals and their positions.
+constexpr int compact_lds_rounds(int N, int CG, int C, int K, int rounds) {
+  const int rows = N - K;
+  const int slots = C - K / CG;
+  const int per = (slots + rounds - 1) / rounds;
+  return rows * (per * CG + 1) + rows + (rows + 3) / 4;
+}
+constexpr int compact_rounds(int N, int L, int CG, int K) {
+  const int C = (N + CG - 1) / CG;
+  const int G = 64 / L;
+  const int cap0 = kCompactBudget / G;
+  const int np = C * CG + 1;
+  const int cap = cap0 > np ? cap0 : np;
+  const int slots = C - K / CG;
+  int r = 1;
+  while (r < slots && compact_lds_rounds(N, CG, C, K, r) > cap) ++r;
+  return r;
+}
+
+// LDS per group: the pivot-row slot (NPAD + 1) or the largest compaction
+// area of the schedule, whichever is larger (they are never in use at once).
+constexpr int group_lds(int N, int L, int CG) {
+  const int RG = L / CG;
+  const int C = (N + CG - 1) / CG;
+  int m = C * CG + 1;
+  int R = (N + RG - 1) / RG;
+  for (;;) {
+    const int K = compact_step(N, RG, R);
+    if (K >= N) break;
+    const int need = compact_lds_rounds(N, CG, C, K, compact_rounds(N, L, CG, K));
+    m = need > m ? need : m;
+    R = (N - K + RG - 1) / RG;
+  }
+  return m | 1;  // odd: the groups' slots start on different LDS banks
+}
+
+// 64-bit mask of the lanes below `lane`.
+__device__ __forceinline__ uint64_t lanes_below(int lane) {
+  return lane >= 64 ? ~0ull : ((1ull << lane) - 1ull);
+}
+
+// Moves the group's active rows from R row slots into R2 < R: the active
+// rows, ordered by (slot, lane-row), become rows j = 0 .. N-K-1 of the new
+// layout, at lane-row j % RG, slot j / RG (the tile's cyclic deal again).
+// Columns below K's slot boundary are finished and not moved.  The rows
+// carry their positions and scales; every element keeps its bits.
+template <int N, int L, int CG, int K, int R, int R2, int C>
+__device__ __forceinline__ void compact(const double2 (&a)[R][C], const RowSlots<R>& rw,
+                                        double2 (&b)[R2][C], RowSlots<R2>& rw2,
+                                        double2* __restrict__ area, int gl, int g) {
+  constexpr int RG = L / CG;
+  constexpr int S0 = K / CG;
+  constexpr int ROWS = N - K;
+  constexpr int NR = compact_rounds(N, L, CG, K);
+  constexpr int SR = (C - S0 + NR - 1) / NR;   // column slots per round
+  constexpr int STRIDE = SR * CG + 1;
+  static_assert(ROWS <= R2 * RG && ROWS > (R2 - 1) * RG, "compaction slot count");
+  const int rb = gl / CG, cb = gl % CG;
+  double2* meta = area + ROWS * STRIDE;                    // (sc, rs) per row
+  int* posv = reinterpret_cast<int*>(meta + ROWS);          // position per row
+  // new row index of each active slot: active rows of earlier slots, then
+  // active lane-rows of this slot below this one (ballot over lane-column 0)
+  const uint64_t gmask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << (g * L));
+  const uint64_t below = lanes_below(g * L + rb * CG);
+  int dst[R];
+  int base = 0;
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const uint64_t m = __ballot(rw.act[t] && cb == 0) & gmask;
+    dst[t] = base + __popcll(m & below);
+    base += __popcll(m);
+  }
+#pragma unroll
+  for (int rd = 0; rd < NR; ++rd) {
+    const int s_lo = S0 + rd * SR;
+    const int s_hi = (s_lo + SR < C) ? s_lo + SR : C;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if (rw.act[t]) {
+#pragma unroll
+        for (int s = s_lo; s < s_hi; ++s) area[dst[t] * STRIDE + cb + (s - s_lo) * CG] = a[t][s];
+        if (rd == 0 && cb == 0) {
+          meta[dst[t]] = make_double2(rw.sc[t], rw.rs[t]);
+          posv[dst[t]] = rw.pos[t];
+        }
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int t2 = 0; t2 < R2; ++t2) {
+      // slots past the last row load a copy of it: they are inactive, so
+      // their (finite) contents never reach a pivot, a multiplier of an
+      // active row or an output
+      const int j = rb + t2 * RG;
+      const bool live = j < ROWS;
+      const int jr = live ? j : ROWS - 1;
+#pragma unroll
+      for (int s = s_lo; s < s_hi; ++s) b[t2][s] = area[jr * STRIDE + cb + (s - s_lo) * CG];
+      if (rd == 0) {
+        const double2 mr = meta[jr];
+        rw2.sc[t2] = live ? mr.x : 1.0;
+        rw2.rs[t2] = live ? mr.y : 1.0;
+        rw2.pos[t2] = live ? posv[jr] : N + j;  // never a position: never the pivot nor k
+        rw2.act[t2] = live;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// One step k of the factorisation over R row slots (see the file comment):
+// pivot-row write and broadcast, det product, multipliers, the reference
+// swap as position bookkeeping, and the update woven with step k+1's search.
 template <int N, int L, int CG, int POLICY, int ARITH, int R, int C, int PR>
-__device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
+__device__ __forceinline__ void factor_step(int k, double2 (&a)[R][C], RowSlots<R>& rw,
+                                            StepState& st, double2* __restrict__ bc, int gl,
+                                            int (&piv_rec)[PR]) {
+  using T = Tile<N, L, CG>;
+  const int cb = gl % CG;
+  const int ck = k % CG, sk = k / CG;
+  bool is_piv[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) is_piv[t] = (rw.pos[t] == st.ppos);
+
+  // The pivot's lane-row writes the pivot row (columns >= k) to LDS.  A
+  // ds_write costs the same however few lanes are active, so:
+  //  * one matrix per wave, or one row slot: one exec-masked region per row
+  //    slot; only the pivot's region runs (empty regions are branched over);
+  //  * several matrices per wave and several slots: their pivots sit in
+  //    different row slots, so the pivot lanes first select their row's slot
+  //    values and one region writes every group's pivot row at once.
+  // The wave executes its LDS instructions in order, so the reads below see
+  // these writes.
+  if constexpr (T::G > 1 && R > 1) {
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) any = any || is_piv[t];
+    if (any) {
+#pragma unroll
+      for (int s = sk; s < C; ++s) {
+        double2 v = a[0][s];
+#pragma unroll
+        for (int t = 1; t < R; ++t) v = sel2(is_piv[t], a[t][s], v);
+        bc[cb + s * CG] = v;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if (is_piv[t]) {
+#pragma unroll
+        for (int s = sk; s < C; ++s) bc[cb + s * CG] = a[t][s];
+      }
+    }
+  }
+  wave_lds_sync();
+  const double2 p = bc[k];
+
+  if (gl == k % L) piv_rec[k / L] = st.ppos;
+  if (st.ppos != k) st.sg = -st.sg;
+
+  // |p|^2 and the determinant product.  Fused arithmetic scales p by
+  // 2^-kp, kp the exponent of max(|Re p|, |Im p|), so that p_s = p 2^-kp has
+  // |p_s| in [0.5, 2^0.5): |p_s|^2 neither overflows nor underflows (it is 0
+  // exactly when p == 0), and the running product of the p_s stays within
+  // 2^+-64 of 1 without renormalisation (det = prod p_s * 2^(sum kp)).  The
+  // scalings are exact, so in the normal range the multipliers have the
+  // bits of the unscaled formula.  Reference arithmetic keeps the plain
+  // product, renormalised every step.
+  double den, psx = p.x, psy = p.y;
+  int kp = 0;
+  if constexpr (ARITH == ARITH_FUSED) {
+    kp = __builtin_amdgcn_frexp_exp(vmax_abs_f64(p.x, p.y));  // 0 for 0, inf, nan
+    psx = ldexp(p.x, -kp);
+    psy = ldexp(p.y, -kp);
+    den = fma(psx, psx, psy * psy);
+    const double nx = fma(st.dx, psx, -(st.dy * psy));
+    const double ny = fma(st.dx, psy, st.dy * psx);
+    st.dx = nx;
+    st.dy = ny;
+    st.de += kp;
+    // keep the exponent sum running: left alone, the compiler holds every
+    // step's kp live and adds them at the end
+    asm volatile("" : "+v"(st.de));
+  } else {
+    den = p.x * p.x + p.y * p.y;
+    const double nx = st.dx * p.x - st.dy * p.y;
+    const double ny = st.dy * p.x + st.dx * p.y;
+    st.dx = nx;
+    st.dy = ny;
+    renorm(st.dx, st.dy, st.de);
+  }
+  const bool zero = (den == 0.0);
+  if (zero && st.info == 0) st.info = k + 1;
+  // the same for the sign and the status
+  asm volatile("" : "+v"(st.sg), "+v"(st.info));
+
+  // multipliers l_t = a_tk / p of this lane's rows, column k broadcast
+  // along the lane-row
+  double lx[R], ly[R];
+  if (k + 1 < N) {
+    double ir = 0.0, ii = 0.0;
+    if constexpr (ARITH == ARITH_FUSED) {
+      // 1/p = conj(p_s) / |p_s|^2 * 2^-kp; a zero pivot gets exactly zero
+      // multipliers: the step changes no row
+      const double rden = recip_scaled(den);
+      ir = ldexp(psx * rden, -kp);
+      ii = ldexp(-(psy * rden), -kp);
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const double ax = lanerow_bcast<CG>(a[t][sk].x, ck);
+      const double ay = lanerow_bcast<CG>(a[t][sk].y, ck);
+      if constexpr (ARITH == ARITH_FUSED) {
+        lx[t] = fma(ax, ir, -(ay * ii));
+        ly[t] = fma(ay, ir, ax * ii);
+      } else {
+        lx[t] = zero ? 0.0 : (ax * p.x + ay * p.y) / den;
+        ly[t] = zero ? 0.0 : (ay * p.x - ax * p.y) / den;
+      }
+    }
+  }
+
+  // the reference's swap, as position bookkeeping
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    if (rw.pos[t] == k) rw.pos[t] = st.ppos;
+    if (is_piv[t]) {
+      rw.pos[t] = k;
+      rw.act[t] = false;
+    }
+  }
+
+  if (k + 1 < N) {
+    // Every row slot is updated, with no divergent branch.  Finished rows
+    // and the pivot row are never read again after this step, so what the
+    // update leaves in them does not matter.  After a zero pivot the
+    // multipliers are exactly zero, and fma(-0, u, e) == e for finite u up
+    // to the sign of a zero element, which no output sees.  (An infinite
+    // entry in such a step's pivot row or column turns into NaN here, where
+    // the reference-order elimination skips the step.)
+    st.ppos = update_and_search<L, CG, POLICY, ARITH>(a, lx, ly, bc, cb, k, rw.pos, rw.act,
+                                                      rw.sc, rw.rs);
+  }
+  wave_lds_sync();
+}
+
+// Steps K0 .. N-1 with the tile in R row slots, compacting to fewer slots
+// at the schedule's next step (compact_step).
+template <int N, int L, int CG, int POLICY, int ARITH, int K0, int R, int C, int PR>
+__device__ __forceinline__ void factor_phase(double2 (&a)[R][C], RowSlots<R>& rw, StepState& st,
+                                             double2* __restrict__ bc, int gl, int g,
+                                             int (&piv_rec)[PR]) {
+  constexpr int RG = L / CG;
+  constexpr int K1c = compact_step(N, RG, R);
+  constexpr int K1 = K1c > K0 ? K1c : N;
+#pragma unroll
+  for (int k = K0; k < K1; ++k) factor_step<N, L, CG, POLICY, ARITH>(k, a, rw, st, bc, gl, piv_rec);
+  if constexpr (K1 < N) {
+    constexpr int R2 = (N - K1 + RG - 1) / RG;
+    double2 b[R2][C];
+    RowSlots<R2> rw2;
+    compact<N, L, CG, K1>(a, rw, b, rw2, bc, gl, g);
+    factor_phase<N, L, CG, POLICY, ARITH, K1>(b, rw2, st, bc, gl, g, piv_rec);
+  }
+}
+
+// Factorisation of one matrix held by a group as a[R][C] (see Tile).  Rows
+// never move between lanes except at compactions; each row slot carries its
+// reference position.  piv_rec[q] receives the pivot position chosen at
+// step gl + q*L.  bc: the group's LDS area (group_lds elements).
+template <int N, int L, int CG, int POLICY, int ARITH, int R, int C, int PR>
+__device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl, int g,
                                                    double2* __restrict__ bc,
                                                    int (&piv_rec)[PR]) {
   using T = Tile<N, L, CG>;
@@ -468,21 +833,19 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
   constexpr int RG = T::RG;
   const int rb = gl / CG, cb = gl % CG;
 
-  int pos[R];
-  bool act[R];
+  RowSlots<R> rw;
 #pragma unroll
   for (int t = 0; t < R; ++t) {
-    pos[t] = rb + t * RG;
-    act[t] = pos[t] < N;
+    rw.pos[t] = rb + t * RG;
+    rw.act[t] = rw.pos[t] < N;
   }
 
   // Row scale s_i = max_j |Re a_ij| (LU-fqhe/src/LUCoreCPU.cu:145-160).
-  double sc[R], rs[R];
   int rej = -1;
 #pragma unroll
   for (int t = 0; t < R; ++t) {
-    sc[t] = 1.0;
-    rs[t] = 1.0;
+    rw.sc[t] = 1.0;
+    rw.rs[t] = 1.0;
   }
   if constexpr (POLICY == POLICY_CROUT_REF) {
     int worst = -1;  // 255 - (first rejected row) over this lane's slots
@@ -493,160 +856,34 @@ __device__ __forceinline__ GroupResult factor_tile(double2 (&a)[R][C], int gl,
       for (int s = 1; s < C; ++s)
         if (cb + s * CG < N) m = vmax_f64(m, fabs(a[t][s].x));
       m = lanerow_max<CG>(m);
-      const bool bad = act[t] && m < kRowScaleMin;
-      if (bad) worst = max(worst, 255 - pos[t]);
-      sc[t] = (act[t] && !bad) ? m : 1.0;
+      const bool bad = rw.act[t] && m < kRowScaleMin;
+      if (bad) worst = max(worst, 255 - rw.pos[t]);
+      rw.sc[t] = (rw.act[t] && !bad) ? m : 1.0;
       // RN(1/s) for quot_rn; NaN sends a row scale above 2^960 (whose
       // reciprocal would not be a normal number) to the division
-      rs[t] = sc[t] <= 0x1p960 ? 1.0 / sc[t] : __builtin_nan("");
+      rw.rs[t] = rw.sc[t] <= 0x1p960 ? 1.0 / rw.sc[t] : __builtin_nan("");
     }
     const int first_bad = group_max_i<L>(worst);
     if (first_bad >= 0) rej = 255 - first_bad;
   }
 
-  double dx = 1.0, dy = 0.0, sg = 1.0;
-  int de = 0, info = 0;
-  int ppos;
+  StepState st{1.0, 0.0, 1.0, 0, 0, -1};
   {
-    SearchState st{-1.0, -1, -1.0, -1};
+    SearchState ss{-1.0, -1, -1.0, -1};
 #pragma unroll
     for (int j = 0; j < search_stages<L>(); ++j)
-      search_stage<L, CG, POLICY, ARITH>(j, st, a, pos, act, sc, rs, cb, 0);
-    ppos = tie_key<POLICY>(st.pm);
+      search_stage<L, CG, POLICY, ARITH>(j, ss, a, rw.pos, rw.act, rw.sc, rw.rs, cb, 0);
+    st.ppos = tie_key<POLICY>(ss.pm);
   }
 
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int ck = k % CG, sk = k / CG;
-    bool is_piv[R];
-#pragma unroll
-    for (int t = 0; t < R; ++t) is_piv[t] = (pos[t] == ppos);
+  factor_phase<N, L, CG, POLICY, ARITH, 0>(a, rw, st, bc, gl, g, piv_rec);
 
-    // The pivot's lane-row writes the pivot row (columns >= k) to LDS.  A
-    // ds_write costs the same however few lanes are active, so:
-    //  * one matrix per wave: one exec-masked region per row slot; only the
-    //    pivot's region runs (empty regions are branched over);
-    //  * several matrices per wave: their pivots sit in different row slots,
-    //    so the pivot lanes first select their row's slot values and one
-    //    region writes every group's pivot row at once.
-    // The wave executes its LDS instructions in order, so the reads below see
-    // these writes.
-    if constexpr (T::G > 1 && R > 1) {
-      bool any = false;
-#pragma unroll
-      for (int t = 0; t < R; ++t) any = any || is_piv[t];
-      if (any) {
-#pragma unroll
-        for (int s = sk; s < C; ++s) {
-          double2 v = a[0][s];
-#pragma unroll
-          for (int t = 1; t < R; ++t) v = sel2(is_piv[t], a[t][s], v);
-          bc[cb + s * CG] = v;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < R; ++t) {
-        if (is_piv[t]) {
-#pragma unroll
-          for (int s = sk; s < C; ++s) bc[cb + s * CG] = a[t][s];
-        }
-      }
-    }
-    wave_lds_sync();
-    const double2 p = bc[k];
-
-    if (gl == k % L) piv_rec[k / L] = ppos;
-    if (ppos != k) sg = -sg;
-
-    // determinant product
-    {
-      double nx, ny;
-      if constexpr (ARITH == ARITH_FUSED) {
-        nx = fma(dx, p.x, -(dy * p.y));
-        ny = fma(dx, p.y, dy * p.x);
-      } else {
-        nx = dx * p.x - dy * p.y;
-        ny = dy * p.x + dx * p.y;
-      }
-      dx = nx;
-      dy = ny;
-      renorm(dx, dy, de);
-    }
-
-    // |p|^2.  Fused arithmetic first scales p by 2^-kp, kp the exponent of
-    // max(|Re p|, |Im p|), so that |p|^2 neither overflows nor underflows:
-    // it is 0 exactly when p == 0.  The scaling is exact and the reciprocal
-    // is scaled back below, so in the normal range the multipliers have the
-    // bits of the unscaled formula.
-    double den, psx = p.x, psy = p.y;
-    int kp = 0;
-    if constexpr (ARITH == ARITH_FUSED) {
-      kp = __builtin_amdgcn_frexp_exp(vmax_abs_f64(p.x, p.y));  // 0 for 0, inf, nan
-      psx = ldexp(p.x, -kp);
-      psy = ldexp(p.y, -kp);
-      den = fma(psx, psx, psy * psy);
-    } else {
-      den = p.x * p.x + p.y * p.y;
-    }
-    const bool zero = (den == 0.0);
-    if (zero && info == 0) info = k + 1;
-
-    // multipliers l_t = a_tk / p of this lane's rows, column k broadcast
-    // along the lane-row
-    double lx[R], ly[R];
-    if (k + 1 < N) {
-      double ir = 0.0, ii = 0.0;
-      if constexpr (ARITH == ARITH_FUSED) {
-        // 1/p = conj(p_s) / |p_s|^2 * 2^-kp; a zero pivot gets exactly zero
-        // multipliers: the step changes no row
-        const double rden = zero ? 0.0 : 1.0 / den;
-        ir = ldexp(psx * rden, -kp);
-        ii = ldexp(-(psy * rden), -kp);
-      }
-#pragma unroll
-      for (int t = 0; t < R; ++t) {
-        const double ax = lanerow_bcast<CG>(a[t][sk].x, ck);
-        const double ay = lanerow_bcast<CG>(a[t][sk].y, ck);
-        if constexpr (ARITH == ARITH_FUSED) {
-          lx[t] = fma(ax, ir, -(ay * ii));
-          ly[t] = fma(ay, ir, ax * ii);
-        } else {
-          lx[t] = zero ? 0.0 : (ax * p.x + ay * p.y) / den;
-          ly[t] = zero ? 0.0 : (ay * p.x - ax * p.y) / den;
-        }
-      }
-    }
-
-    // the reference's swap, as position bookkeeping
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-      if (pos[t] == k) pos[t] = ppos;
-      if (is_piv[t]) {
-        pos[t] = k;
-        act[t] = false;
-      }
-    }
-
-    if (k + 1 < N) {
-      // Every row slot is updated, with no divergent branch.  Finished rows
-      // and the pivot row are never read again after this step, so what the
-      // update leaves in them does not matter.  After a zero pivot the
-      // multipliers are exactly zero (ir = ii = 0 in fused arithmetic, an
-      // explicit zero in reference arithmetic), and fma(-0, u, e) == e for
-      // finite u up to the sign of a zero element, which no output sees.
-      // (An infinite entry in such a step's pivot row or column turns into
-      // NaN here, where the reference-order elimination skips the step.)
-      ppos = update_and_search<L, CG, POLICY, ARITH>(a, lx, ly, bc, cb, k, pos, act, sc, rs);
-    }
-    wave_lds_sync();
-  }
-
+  if constexpr (ARITH == ARITH_FUSED) renorm(st.dx, st.dy, st.de);
   GroupResult res;
-  res.mx = sg * dx;
-  res.my = sg * dy;
-  res.e = de;
-  res.info = (rej >= 0) ? -(rej + 1) : info;
+  res.mx = st.sg * st.dx;
+  res.my = st.sg * st.dy;
+  res.e = st.de;
+  res.info = (rej >= 0) ? -(rej + 1) : st.info;
   return res;
 }
 
@@ -720,7 +957,7 @@ __device__ __forceinline__ void det_task(const BatchArgs& args, int64_t task,
                                          double2* __restrict__ bc_wave, int lane) {
   using T = Tile<N, L, CG>;
   constexpr int R = T::R, C = T::C, RG = T::RG, G = T::G;
-  constexpr int NP = T::NPAD + 1;  // padded LDS slot: groups on distinct banks
+  constexpr int NP = group_lds(N, L, CG);
   constexpr int PR = (N + L - 1) / L;
   const int g = lane / L;
   const int gl = lane % L;
@@ -778,7 +1015,7 @@ __device__ __forceinline__ void det_task(const BatchArgs& args, int64_t task,
   int piv_rec[PR];
 #pragma unroll
   for (int q = 0; q < PR; ++q) piv_rec[q] = -1;
-  GroupResult res = factor_tile<N, L, CG, POLICY, ARITH>(a, gl, bc, piv_rec);
+  GroupResult res = factor_tile<N, L, CG, POLICY, ARITH>(a, gl, g, bc, piv_rec);
   if (args.row_perm && group_max_i<L>(bad_perm) > 0) res.info = kInfoBadPerm;
 
   if (mat_ok) {
@@ -799,7 +1036,7 @@ template <int N, int L, int CG, int POLICY, int ARITH, int WPB, int OCC>
 __global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) {
   using T = Tile<N, L, CG>;
   constexpr int G = T::G;
-  constexpr int NP = T::NPAD + 1;
+  constexpr int NP = group_lds(N, L, CG);
   __shared__ double2 bcast[WPB * G * NP];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -813,12 +1050,13 @@ __global__ void __launch_bounds__(64 * WPB, OCC) lup_det_kernel(BatchArgs args) 
 // compiled; the autotuner picks one per n on the device and candidate 0 is
 // the default before tuning.  The lists come from shape sweeps on MI355X:
 // small n wants many matrices per wave (even two rows per lane), mid n a
-// split of each row over lane-columns, n > 40 one matrix per wave.  From
-// n = 41 the tile alone needs 100-256 VGPRs and the natural allocation
-// leaves one wave per SIMD, which cannot hide the step's dependent
-// latencies; the waves = 2 candidates cap the allocation at 256 registers
-// (spilling the coldest values to scratch), which measured up to 1.6x
-// faster at n = 48.
+// split of each row over lane-columns, n > 40 one matrix per wave.  Up to
+// n = 40 the natural allocation sits just under or just over 256 VGPRs, so
+// those candidates are capped at 256 (two waves per SIMD).  From n = 41 the
+// tile alone needs 100-256 VGPRs and the natural allocation leaves one wave
+// per SIMD, which cannot hide the step's dependent latencies; the waves = 2
+// candidates cap the allocation at 256 registers (spilling the coldest
+// values to scratch), which measured up to 1.6x faster at n = 48.
 struct ShapeLC {
   int lanes;
   int lane_cols;
@@ -836,15 +1074,15 @@ constexpr ShapeLC candidate_shape(int n, int idx) {
     return c[idx];
   }
   if (n <= 24) {
-    constexpr ShapeLC c[] = {{32, 4, 1}, {16, 2, 1}, {32, 2, 1}, none};
+    constexpr ShapeLC c[] = {{32, 4, 2}, {16, 2, 2}, {32, 2, 2}, none};
     return c[idx];
   }
   if (n <= 32) {
-    constexpr ShapeLC c[] = {{32, 2, 1}, {32, 4, 1}, {32, 1, 1}, none};
+    constexpr ShapeLC c[] = {{32, 2, 2}, {32, 4, 2}, {32, 1, 2}, none};
     return c[idx];
   }
   if (n <= 40) {
-    constexpr ShapeLC c[] = {{64, 4, 1}, {64, 1, 1}, {64, 4, 2}, none};
+    constexpr ShapeLC c[] = {{64, 4, 2}, {64, 1, 2}, {64, 2, 2}, none};
     return c[idx];
   }
   if (n <= 48) {

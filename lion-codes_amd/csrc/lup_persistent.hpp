@@ -51,11 +51,10 @@ constexpr PersistShape persist_shape(int n) {
 }
 constexpr int persist_group(int n) { return 64 / persist_shape(n).lanes; }  // matrices per task
 
-// LDS broadcast slots one task of size n uses (G groups of NPAD + 1).
+// LDS one task of size n uses (G groups of group_lds).
 constexpr int persist_lds(int n) {
   const PersistShape s = persist_shape(n);
-  const int cols = (n + s.lane_cols - 1) / s.lane_cols;
-  return (64 / s.lanes) * (cols * s.lane_cols + 1);
+  return (64 / s.lanes) * group_lds(n, s.lanes, s.lane_cols);
 }
 constexpr int persist_lds_max(int lo, int hi) {
   int m = 0;

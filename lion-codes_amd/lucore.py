@@ -125,6 +125,11 @@ def load() -> ctypes.CDLL:
                                          _i32]
     lib.lufqhe_cpu_pair_sum.restype = _i32
     lib.lufqhe_cpu_pair_sum.argtypes = [_vp, _i64, _i32, _i64, _i64, _vp, _vp, _i32]
+    lib.lufqhe_mc_split.restype = _i32
+    lib.lufqhe_mc_split.argtypes = [_i64, _i32, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
+    lib.lufqhe_mc_lu_run.restype = _i32
+    lib.lufqhe_mc_lu_run.argtypes = [_vp, _i64, _i32, _i32, _i32, _i32, _vp,
+                                     ctypes.POINTER(McStats)]
     lib.lufqhe_fqhe_build.restype = _i32
     lib.lufqhe_fqhe_build.argtypes = [_vp, _i64, _i32, _i64, _vp, _vp, _vp]
     _lib = lib
@@ -406,6 +411,37 @@ def cpu_pair_sum(A, nthreads: int = 0) -> tuple[complex, int]:
     if st != SUCCESS:
         raise LufqheError("lufqhe_cpu_pair_sum", st)
     return complex(out[0]), bad.value
+
+
+class McStats(ctypes.Structure):
+    """lufqhe_mc_stats."""
+    _fields_ = [("gpu_matrices", _i64), ("cpu_matrices", _i64), ("wall_s", ctypes.c_double),
+                ("gpu_s", ctypes.c_double), ("cpu_s", ctypes.c_double)]
+
+
+def mc_split(nelem: int, cpu_count: int) -> tuple[int, int]:
+    """(GPU matrices, host matrices) of the reference's work split."""
+    g, c = _i64(), _i64()
+    st = load().lufqhe_mc_split(nelem, cpu_count, ctypes.byref(g), ctypes.byref(c))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_mc_split", st)
+    return g.value, c.value
+
+
+def mc_lu_run(A, cpu_count: int = 16, iterations: int = 1, policy: str = "crout_ref"):
+    """The reference's Monte Carlo loop in LU mode on one GPU (lufqhe_mc_lu_run):
+    ``A`` a numpy complex128 array [nelem, n, n] on the host.  Returns
+    (results complex128 [iterations], stats dict)."""
+    import numpy as np
+    A = np.ascontiguousarray(A, np.complex128)
+    nelem, n, _ = A.shape
+    out = np.empty(iterations, np.complex128)
+    stats = McStats()
+    st = load().lufqhe_mc_lu_run(A.ctypes.data, nelem, n, cpu_count, iterations,
+                                 POLICIES[policy], out.ctypes.data, ctypes.byref(stats))
+    if st != SUCCESS:
+        raise LufqheError("lufqhe_mc_lu_run", st)
+    return out, {name: getattr(stats, name) for name, _ in McStats._fields_}
 
 
 class ServeConfig(ctypes.Structure):

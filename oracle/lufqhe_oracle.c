@@ -349,23 +349,12 @@ int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
         if (piv) piv[k] = ppos;
         const double px = a[pr][k][0], py = a[pr][k][1];
         if (ppos != k) sg = -sg;
-        /* determinant product */
-        double nx, ny;
-        if (arith == ORACLE_ARITH_FUSED) {
-            double t = dy * py;
-            nx = fma(dx, px, -t);
-            t = dy * px;
-            ny = fma(dx, py, t);
-        } else {
-            nx = dx * px - dy * py;
-            ny = dy * px + dx * py;
-        }
-        dx = nx;
-        dy = ny;
-        renorm(&dx, &dy, &de);
-        /* multipliers and rank-1 trailing update; fused arithmetic scales p
-         * by 2^-kp before squaring (kp the exponent of max(|Re p|, |Im p|)),
-         * so |p|^2 is 0 only for p == 0, and scales the reciprocal back */
+        /* determinant product and |p|^2.  Fused arithmetic scales p by
+         * 2^-kp (kp the exponent of max(|Re p|, |Im p|)): |p_s|^2 is 0 only
+         * for p == 0, the reciprocal is scaled back below, and the running
+         * product of the p_s (within 2^+-64 of 1) is renormalised once at
+         * the end, det = prod p_s * 2^(sum kp).  Reference arithmetic keeps
+         * the plain product, renormalised every step. */
         double den;
         int kp = 0;
         double psx = px, psy = py;
@@ -374,8 +363,20 @@ int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
             psx = ldexp(px, -kp);
             psy = ldexp(py, -kp);
             den = fma(psx, psx, psy * psy);
+            double t = dy * psy;
+            const double nx = fma(dx, psx, -t);
+            t = dy * psx;
+            const double ny = fma(dx, psy, t);
+            dx = nx;
+            dy = ny;
+            de += kp;
         } else {
             den = px * px + py * py;
+            const double nx = dx * px - dy * py;
+            const double ny = dy * px + dx * py;
+            dx = nx;
+            dy = ny;
+            renorm(&dx, &dy, &de);
         }
         if (den == 0.0) {
             if (!info) info = k + 1;
@@ -421,6 +422,8 @@ int oracle_lup_mirror_z(const double *A, int n, int ld, int policy, int arith,
         pos[pr] = k;
         act[pr] = 0;
     }
+    if (arith == ORACLE_ARITH_FUSED) renorm(&dx, &dy, &de);
+    if (dx == 0.0 && dy == 0.0) de = 0; /* a zero determinant is (0, 2^0) */
     if (mant) {
         mant[0] = sg * dx;
         mant[1] = sg * dy;

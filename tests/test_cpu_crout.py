@@ -92,3 +92,13 @@ def test_cpp_lucore_cpu_api(lc, orc, wl):
     s = complex(vals[20], vals[21])
     assert bits(np.array([s])).tolist() == bits(np.array([orc.crout_ref_pairsum(A)])).tolist()
     assert vals[22] == 1.0    # the rejected-row case threw LUCoreCPU::Error
+
+
+def test_mc_split_follows_the_reference(lc):
+    """LU-fqhe/src/MonteCarloLoop.cu:139-148 with one GPU: the GPU counts as
+    32 host cores and its share is even."""
+    assert lc.mc_split(2000, 8) == (1600, 400)       # 2000 / 1.25
+    assert lc.mc_split(1001, 0) == (1000, 1)         # all but the odd one
+    assert lc.mc_split(1000, 32) == (500, 500)
+    assert lc.mc_split(10, 16) == (6, 4)             # floor(10 / 1.5) = 6
+    assert lc.mc_split(0, 16) == (0, 0)

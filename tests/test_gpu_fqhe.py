@@ -65,11 +65,15 @@ def test_config3_analytic_log_form(lc, orc, wl, dev, policy):
     The monomial matrices of 10..40 random points in a disk are ill
     conditioned: rounding the entries to FP64 alone moves log det by more
     than 1e-10 relative for a few matrices in a thousand (a long-double LU of
-    the same FP64 matrices misses 1e-10 on 58 of these 32768, LAPACK's zgetrf
-    on 97).  So the analytic check is: 99.5% within 1e-10, and every matrix
-    within 1000x the error of numpy's LAPACK slogdet on the same matrix (or
-    1e-12).  Parity is the bit-exact check against the mirror oracle on a
-    sample of device-built matrices."""
+    the same FP64 matrices misses 1e-10 against the analytic form on 58 of
+    these 32768, LAPACK's zgetrf on 97).  So the checks are: 99.5% within
+    1e-10 of the analytic form; and, against the long-double LU of the same
+    device-built matrices, the engine's error stays within the conditioning
+    that LAPACK's own error on the same matrix shows -- at most 50x it (or
+    1e-13) for 99.9% of the matrices and 700x (or 1e-12) for all (measured
+    with the oracle's mirror: 36x at the 99.9th percentile, 351x at most).
+    Parity is the bit-exact check against the mirror oracle on a sample of
+    device-built matrices."""
     z, ns, offs = wl.fqhe_configs(32768, 10, 40, seed=3)
     plan = lc.MixedPlan(ns, offs, policy=policy, nstreams=4)
     res = lc.alloc_result(len(ns), 64, dev)
@@ -89,18 +93,24 @@ def test_config3_analytic_log_form(lc, orc, wl, dev, policy):
     err = log_err(la, ph, la_ref, ph_ref)
     assert (err <= 1e-10).mean() >= 0.995, (err <= 1e-10).mean()
     assert np.median(err) < 1e-14
-    # per n: device-built matrices -> LAPACK error bound and a bit-exact sample
+    # per n: device-built matrices -> error against their long-double LU,
+    # bounded by LAPACK's error on the same matrices, and a bit-exact sample
     rng = np.random.default_rng(1)
+    e_ld = np.empty(len(ns))
+    e_lap = np.empty(len(ns))
     for n in np.unique(ns):
         idx = np.nonzero(ns == n)[0]
         zz = torch.from_numpy(np.stack([z[offs[m]:offs[m] + n] for m in idx])).to(dev)
         A = lc.fqhe_build(zz).cpu().numpy()
+        t = orc.truth(A)
         s, l = np.linalg.slogdet(A)
-        e_lap = log_err(l, np.angle(s), la_ref[idx], ph_ref[idx])
-        assert np.all(err[idx] <= np.maximum(1000.0 * e_lap, 1e-12)), n
+        e_ld[idx] = log_err(la[idx], ph[idx], t["logabs"], t["phase"])
+        e_lap[idx] = log_err(l, np.angle(s), t["logabs"], t["phase"])
         pick = rng.choice(len(idx), min(len(idx), 40), replace=False)
         o = orc.mirror(A[pick], policy, "fused", pivots=False)
         assert np.array_equal(bits(mant[idx[pick]]), bits(o["mant"])), n
+    assert np.mean(e_ld <= np.maximum(50.0 * e_lap, 1e-13)) >= 0.999
+    assert np.all(e_ld <= np.maximum(700.0 * e_lap, 1e-12)), float(np.max(e_ld / np.maximum(e_lap, 1e-16)))
     plan.close()
 
 
