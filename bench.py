@@ -42,13 +42,33 @@ FP64_PEAK_TFLOPS = 78.6
 OUT_BYTES = 16 + 4 + 8 + 8 + 4   # mant, exp, logabs, phase, info
 # HBM bytes one launch of the n=32 kernel moves, measured with rocprofv3
 # (FETCH_SIZE and WRITE_SIZE, one counter per run, each scaled by what the
-# same counter reads for a plain 1 GiB device copy: x2.0 and x1.0).
-MEASURED_TRAFFIC_N32 = 1.0836e9
+# same counter reads for a plain 1 GiB device copy: x2.0 and x1.0;
+# profiles/lup_det_kernel_hbm_counters.md).
+MEASURED_TRAFFIC_N32 = 1.0799e9
+# FP64 instructions one launch of the n=32 kernel issues, per wave (a wave
+# factors two matrices; 32768 waves per launch), measured with rocprofv3
+# (SQ_INSTS_VALU_FMA_F64, _MUL_F64, _ADD_F64 and SQ_INSTS_VALU;
+# profiles/lup_det_kernel_sizes_sq_counters.md).  Every one runs on all 64
+# lanes, so the FP64 flop the hardware executes per launch is
+# 32768 x 64 x (2 FMA + MUL + ADD).
+MEASURED_F64_N32 = {"waves": 32768, "fma": 2317.0, "mul": 318.0, "add": 48.0, "valu": 6167.0}
 DEVICE_WARM_S = 0.25  # untimed device warm-up before the W warm-up steps
 
 
 def det_bytes(n: int) -> int:
     return 16 * n * n + OUT_BYTES
+
+
+def fp64_counter(kern_ms: float) -> dict:
+    """The counter twin of fp64_frac: FP64 flop the n=32 kernel executes per
+    launch (rocprofv3 instruction counts, all 64 lanes) over the live kernel
+    time, and the FP64 instructions' share of the VALU instructions."""
+    m = MEASURED_F64_N32
+    flop = m["waves"] * 64 * (2 * m["fma"] + m["mul"] + m["add"])
+    tflops = flop / (kern_ms * 1e-3) / 1e12
+    return {"executed_flop_per_launch": flop, "tflops": tflops, "frac": tflops / FP64_PEAK_TFLOPS,
+            "f64_share_of_valu": (m["fma"] + m["mul"] + m["add"]) / m["valu"],
+            "source": "profiles/lup_det_kernel_sizes_sq_counters.md"}
 
 
 def det_flops(n: int) -> float:
@@ -514,6 +534,7 @@ def main() -> None:
                      "kernel_ms": kern_ms,
                      "fp64_tflops": flops_launch / (kern_ms * 1e-3) / 1e12,
                      "fp64_frac": flops_launch / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                     "fp64_counter": fp64_counter(kern_ms),
                      "bytes_per_det": det_bytes(N), "flops_per_det": det_flops(N)},
     }
     samples = {}
