@@ -20,15 +20,17 @@
 //      this search is woven into step k-1's update (update_and_search);
 //   3. the pivot's lane-row writes the pivot row (columns k..n-1) to the
 //      group's LDS slot and every lane reads its columns back;
-//   4. det *= pivot (complex, renormalised to mant * 2^exp);
+//   4. det *= pivot (ARITH_FUSED: the pivot scaled by a power of two, the
+//      exponents summed, one renormalisation to mant * 2^exp at the end);
 //   5. multipliers l = a_ik / p (lane-row DPP broadcast of column k) and the
 //      rank-1 update of columns k+1..n-1 (complex FP64, 4 FMAs per element in
 //      ARITH_FUSED);
 //   6. the position bookkeeping of the reference swap; the pivot row retires.
 // The k loop is fully unrolled (N is a template constant) so that finished
-// column slots are skipped statically and the tile stays in registers.  The
-// load phase reads the matrix from HBM, or builds an FQHE matrix from
-// particle positions (SRC_FQHE).
+// column slots are skipped statically and the tile stays in registers; once
+// the active rows fit in fewer row slots the tile is compacted into them
+// (compact).  The load phase reads the matrix from HBM, or builds an FQHE
+// matrix from particle positions (SRC_FQHE).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -173,6 +173,27 @@ def test_pivot_rule_is_scaled_real_part_with_last_tie(orc):
     assert m3["piv"][0, 0] == 0  # 4 > 3 although |2+2i| < |3i|: cabs1, not the modulus
 
 
+def test_abs_pivots_equal_lapack_zgetrf(orc, wl):
+    """The ABS policy is LAPACK's partial pivoting: scipy's lu_factor (zgetrf,
+    izamax over cabs1, first index on ties) interchanges the same rows at every
+    step.  N(0,1) matrices agree at every step; small-integer matrices, whose
+    first column is full of exact cabs1 ties, agree at step 0 every time (the
+    later steps of a few of them differ because zgetrf's blocked elimination
+    rounds the updates differently at ties)."""
+    from scipy.linalg import lu_factor
+    for n, count in ((5, 600), (16, 400), (32, 200), (64, 60)):
+        A = wl.gaussian_np(count, n, seed=900 + n)
+        piv = orc.mirror(A, "abs", "fused")["piv"]
+        lap = np.stack([lu_factor(a, check_finite=False)[1] for a in A])
+        assert np.array_equal(piv, lap), n
+    rng = np.random.default_rng(5)
+    A = (rng.integers(-2, 3, (1000, 8, 8)) + 1j * rng.integers(-2, 3, (1000, 8, 8))).astype(complex)
+    piv = orc.mirror(A, "abs", "fused")["piv"]
+    lap = np.stack([lu_factor(a, check_finite=False)[1] for a in A])
+    assert np.array_equal(piv[:, 0], lap[:, 0])
+    assert np.mean(np.all(piv == lap, axis=1)) > 0.98
+
+
 def test_fused_criterion_keeps_non_power_of_two_ties(orc):
     """Both rows of column 0 have the criterion 1 exactly (their max |Re| is in
     column 0), so the reference's `>=` picks the last row.  49 * RN(1/49)
