@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include "lup_kernel.hpp"
+#include "scaled_sum.hpp"
 
 namespace lufqhe {
 
@@ -29,34 +30,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
 }
 
-// (mant, exp) number: value = m * 2^e; zero has m == 0.
-struct Scaled {
-  double2 m;
-  int e;
-};
-
-__device__ __forceinline__ Scaled scaled_norm(double2 m, int e) {
-  const double a = fmax(fabs(m.x), fabs(m.y));
-  if (a == 0.0) return Scaled{make_double2(0.0, 0.0), 0};
-  const int k = __builtin_amdgcn_frexp_exp(a);
-  return Scaled{make_double2(ldexp(m.x, -k), ldexp(m.y, -k)), e + k};
-}
-
-// a + b, aligned to the larger exponent; the smaller term is scaled exactly
-// unless it drops below the double range, where it is negligible anyway.
-__device__ __forceinline__ Scaled scaled_add(Scaled a, Scaled b) {
-  if (a.m.x == 0.0 && a.m.y == 0.0) return b;
-  if (b.m.x == 0.0 && b.m.y == 0.0) return a;
-  if (a.e < b.e) {
-    const Scaled t = a;
-    a = b;
-    b = t;
-  }
-  const int d = a.e - b.e;
-  const double2 bs = d > 1100 ? make_double2(0.0, 0.0)
-                              : make_double2(ldexp(b.m.x, -d), ldexp(b.m.y, -d));
-  return scaled_norm(make_double2(a.m.x + bs.x, a.m.y + bs.y), a.e);
-}
+// (mant, exp) sums: Scaled, scaled_norm, scaled_add (scaled_sum.hpp).
 
 // ---------------------------------------------------------------------------
 // In-place write-back of plain determinants into A_m[0][0].

@@ -39,6 +39,7 @@
 
 #include "../../include/lufqhe.h"
 #include "lup_dispatch.hpp"
+#include "scaled_sum.hpp"
 
 namespace lufqhe {
 int32_t hip_fail(hipError_t e);
@@ -92,30 +93,7 @@ __global__ void __launch_bounds__(256) serve_prepare_kernel(
   }
 }
 
-// (mant, exp) arithmetic for the per-request sums.
-struct Sc {
-  double2 m;
-  int e;
-};
-__device__ __forceinline__ Sc sc_norm(double2 m, int e) {
-  const double a = fmax(fabs(m.x), fabs(m.y));
-  if (a == 0.0) return Sc{make_double2(0.0, 0.0), 0};
-  const int k = __builtin_amdgcn_frexp_exp(a);
-  return Sc{make_double2(ldexp(m.x, -k), ldexp(m.y, -k)), e + k};
-}
-__device__ __forceinline__ Sc sc_add(Sc a, Sc b) {
-  if (a.m.x == 0.0 && a.m.y == 0.0) return b;
-  if (b.m.x == 0.0 && b.m.y == 0.0) return a;
-  if (a.e < b.e) {
-    const Sc t = a;
-    a = b;
-    b = t;
-  }
-  const int d = a.e - b.e;
-  const double2 bs = d > 1100 ? make_double2(0.0, 0.0)
-                              : make_double2(ldexp(b.m.x, -d), ldexp(b.m.y, -d));
-  return sc_norm(make_double2(a.m.x + bs.x, a.m.y + bs.y), a.e);
-}
+// The per-request sums use the (mant, exp) arithmetic of scaled_sum.hpp.
 
 // One 64-thread workgroup per request: lane l adds terms l, l+64, ... in
 // order, then a fixed tree over the lanes.  psi' = out_m * 2^out_e.
@@ -128,17 +106,17 @@ __global__ void __launch_bounds__(64) serve_sum_kernel(const ReqDesc* __restrict
   __shared__ int se[64];
   const ReqDesc r = reqs[blockIdx.x];
   const int l = threadIdx.x;
-  Sc acc{make_double2(0.0, 0.0), 0};
+  Scaled acc{make_double2(0.0, 0.0), 0};
   for (int t = l; t < r.k; t += 64) {
     const int64_t i = r.out_base + t;
-    acc = sc_add(acc, sc_norm(mant[i], expo[i]));
+    acc = scaled_add(acc, scaled_norm(mant[i], expo[i]));
   }
   sm[l] = acc.m;
   se[l] = acc.e;
   __syncthreads();
   for (int w = 32; w > 0; w >>= 1) {
     if (l < w) {
-      const Sc s = sc_add(Sc{sm[l], se[l]}, Sc{sm[l + w], se[l + w]});
+      const Scaled s = scaled_add(Scaled{sm[l], se[l]}, Scaled{sm[l + w], se[l + w]});
       sm[l] = s.m;
       se[l] = s.e;
     }
