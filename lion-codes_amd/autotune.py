@@ -9,7 +9,7 @@ per (policy, n) to a JSON table.  The Python binding applies the table when
 it loads the library (lucore.SHAPE_TABLE); C and C++ callers apply it with
 lufqhe_load_shape_table, which LUCoreGPU::CALL does on first use.
 
-    python lion-codes_amd/autotune.py [--out PATH] [--ns 8,16,32] [--reps 5]
+    python lion-codes_amd/autotune.py [--out PATH] [--ns 8,16,32] [--reps 5] [--count N]
 """
 from __future__ import annotations
 
@@ -44,7 +44,7 @@ def time_shape(lc, A, res, policy: str, n: int, shape: int, reps: int) -> float:
 
 
 def tune(policies=("crout_ref", "abs"), ns=range(1, 65), target_mib: int = 256, reps: int = 5,
-         seed: int = 4242, log=print) -> dict:
+         seed: int = 4242, log=print, count: int = 0) -> dict:
     lc = importlib.import_module("lion-codes_amd")
     wl = importlib.import_module("lion-codes_amd.workloads")
     lc.load()
@@ -62,8 +62,8 @@ def tune(policies=("crout_ref", "abs"), ns=range(1, 65), target_mib: int = 256, 
     entries = []
     for policy in policies:
         for n in ns:
-            count = int(min(262144, max(4096, (target_mib << 20) // (16 * n * n))))
-            A = wl.gaussian(count, n, seed + n, device=dev)
+            cnt = count or int(min(262144, max(4096, (target_mib << 20) // (16 * n * n))))
+            A = wl.gaussian(cnt, n, seed + n, device=dev)
             res = lc.alloc_result(count, n, dev)
             ref = None
             times = []
@@ -80,9 +80,9 @@ def tune(policies=("crout_ref", "abs"), ns=range(1, 65), target_mib: int = 256, 
             lc.set_shape(policy, "fused", n, best)
             lanes, cols, waves = lc.shape_info(n, best)
             entries.append({"policy": policy, "n": n, "shape": best, "lanes": lanes,
-                            "lane_cols": cols, "waves": waves, "count": count,
+                            "lane_cols": cols, "waves": waves, "count": cnt,
                             "us_per_launch": [round(t * 1e3, 2) for t in times],
-                            "ns_per_det": round(times[best] * 1e6 / count, 3)})
+                            "ns_per_det": round(times[best] * 1e6 / cnt, 3)})
             log(f"{policy:9s} n={n:2d} shape={best} (L={lanes}, CG={cols}, waves={waves}) "
                 f"times(us)={[round(t * 1e3, 1) for t in times]}")
             del A, res
@@ -98,13 +98,16 @@ def main() -> None:
     ap.add_argument("--ns", default="1-64")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--mib", type=int, default=256)
+    ap.add_argument("--count", type=int, default=0,
+                    help="matrices per size (default: about --mib MiB, at least 4096)")
     args = ap.parse_args()
     if "-" in args.ns:
         lo, hi = map(int, args.ns.split("-"))
         ns = range(lo, hi + 1)
     else:
         ns = [int(x) for x in args.ns.split(",")]
-    table = tune(ns=ns, target_mib=args.mib, reps=args.reps, log=lambda m: print(m, flush=True))
+    table = tune(ns=ns, target_mib=args.mib, reps=args.reps, log=lambda m: print(m, flush=True),
+                 count=args.count)
     Path(args.out).write_text(json.dumps(table, indent=1))
     print(f"wrote {args.out} ({len(table['entries'])} entries)", flush=True)
 

@@ -292,6 +292,21 @@ def test_full_size_pivots_equal_reference_crout(lc, orc, wl, dev, n, count, seed
         np.median(e), e.max(), int((e > 1e-12).sum()))
 
 
+@pytest.mark.parametrize("n,count", [(8, 2048), (16, 2048), (32, 1024), (48, 256), (64, 256)])
+def test_abs_pivots_equal_lapack(lc, wl, dev, n, count):
+    """Under the ABS policy the kernels interchange the rows LAPACK's zgetrf
+    does (scipy's lu_factor: izamax over cabs1, first index on ties) at every
+    step of N(0,1) matrices."""
+    from scipy.linalg import lu_factor
+    A = wl.gaussian(count, n, seed=300 + n, device=dev)
+    r = lc.det_batched(A, pivots=True, policy="abs")
+    torch.cuda.synchronize()
+    piv = r.piv.cpu().numpy().astype(np.int64)
+    lap = np.stack([lu_factor(a, check_finite=False)[1] for a in A.cpu().numpy()])
+    bad = np.nonzero(np.any(piv != lap, axis=1))[0]
+    assert len(bad) == 0, (n, bad[:8])
+
+
 def test_mixed_sample_pivots_equal_reference_crout(lc, orc, wl, dev):
     """The same pivot parity on a sample of the mixed n = 8..64 batch with 5%
     near-singular matrices (each size through its tuned kernel, which the
