@@ -275,7 +275,8 @@ def det_inplace(A: torch.Tensor, pair_product: bool = True, policy: str = "crout
                 stream: torch.cuda.Stream | None = None) -> None:
     """The reference's in-place contract (LUCoreGPU::CALL with WRITE_DET and
     PERFORM_PROD): A[m,0,0] = det(A[m]), A[2i,0,0] = det(A[2i]) * det(A[2i+1])."""
-    if A.dtype != torch.complex128 or not A.is_cuda or A.dim() != 3 or A.stride(2) != 1:
+    if (A.dtype != torch.complex128 or not A.is_cuda or A.dim() != 3 or A.stride(2) != 1
+            or A.shape[1] != A.shape[2]):
         raise ValueError("A must be a complex128 CUDA tensor [B, n, n] with unit column stride")
     B, n, _ = A.shape
     _check_workspace(workspace, load().lufqhe_det_inplace_workspace_bytes(B), A.device)
@@ -403,6 +404,8 @@ def cpu_pair_sum(A, nthreads: int = 0) -> tuple[complex, int]:
     rejected row (-1 if none)."""
     import numpy as np
     A = np.ascontiguousarray(A, np.complex128)
+    if A.ndim != 3 or A.shape[1] != A.shape[2]:
+        raise ValueError("A must be a complex128 array [B, n, n]")
     B, n, _ = A.shape
     out = np.empty(1, np.complex128)
     bad = _i32()
@@ -434,6 +437,8 @@ def mc_lu_run(A, cpu_count: int = 16, iterations: int = 1, policy: str = "crout_
     (results complex128 [iterations], stats dict)."""
     import numpy as np
     A = np.ascontiguousarray(A, np.complex128)
+    if A.ndim != 3 or A.shape[1] != A.shape[2]:
+        raise ValueError("A must be a complex128 array [nelem, n, n]")
     nelem, n, _ = A.shape
     out = np.empty(iterations, np.complex128)
     stats = McStats()

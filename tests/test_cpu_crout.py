@@ -102,3 +102,15 @@ def test_mc_split_follows_the_reference(lc):
     assert lc.mc_split(1000, 32) == (500, 500)
     assert lc.mc_split(10, 16) == (6, 4)             # floor(10 / 1.5) = 6
     assert lc.mc_split(0, 16) == (0, 0)
+
+
+def test_host_bindings_refuse_non_square_batches(lc):
+    """A batch whose matrices are not square is refused before any C call
+    reads n*n elements per matrix."""
+    A = np.zeros((4, 6, 5), complex)
+    with pytest.raises(ValueError):
+        lc.cpu_pair_sum(A)
+    with pytest.raises(ValueError):
+        lc.mc_lu_run(A, cpu_count=16, iterations=1)
+    with pytest.raises(ValueError):
+        lc.cpu_crout_det(A)
